@@ -1,0 +1,37 @@
+/*
+ * voce_prof.h -- kernel timing for measurement tools.
+ *
+ * No counterpart in the reference (it has no timers, see evolution.c:344-348
+ * for its only progress output).  For benchmarks that must report a
+ * kernel's own average duration: HIP events are recorded on the context's
+ * stream between the five launches of every FFT iteration, and only
+ * iterations that did work (before convergence) are accumulated.
+ */
+#ifndef VOCE_PROF_H
+#define VOCE_PROF_H
+
+#include "voce.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernels of one FFT iteration, in launch order */
+#define VOCE_PROF_PLANE_FWD 0   /* z r2c + y FFT of the stress */
+#define VOCE_PROF_XGREEN 1      /* x FFT, Green operator, inverse x FFT */
+#define VOCE_PROF_PLANE_INV 2   /* inverse y FFT, z c2r, strain update */
+#define VOCE_PROF_NR 3          /* per-voxel Newton-Raphson + partial sums */
+#define VOCE_PROF_FIN 4         /* single-block finaliser */
+#define VOCE_PROF_NK 5
+
+/* Turn timing on (1) or off (0); either way the totals are reset. */
+int voce_prof_enable(voce_ctx *ctx, int on);
+/* Totals since the last enable: ms_total[k] milliseconds over count[k] launches. */
+int voce_prof_read(voce_ctx *ctx, double ms_total[VOCE_PROF_NK], long count[VOCE_PROF_NK]);
+/* out[0..5]: voxels, nx, ny, nz, precision, two-CRSS flag */
+int voce_ctx_info(const voce_ctx *ctx, long out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

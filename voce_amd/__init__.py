@@ -1,0 +1,252 @@
+"""voce_amd -- MI355X EVP-FFT crystal-plasticity solver (Python view of the C ABI).
+
+The solver itself is the native library ``voce_amd/libvoce.so`` (HIP kernels
+for gfx950 behind the C ABI of ``include/voce.h``).  This module binds that
+ABI with ctypes; it holds no solver logic and has no fallback: if the
+library is missing, every entry point raises.
+
+The Python names follow the reference's boundary (SetupJob / Evolution /
+DestroyJob, init.c:617-930 and evolution.c:265-622):
+
+    case = load_case("input")               # GetNameList + .sx + mesh
+    s = Solver(case.setup, precision="fp64")  # SetupJob
+    rep = s.increment()                     # one istep of Evolution
+    sig = s.field("sig")                    # (6, V) like WriteSigMPI's data
+    s.close()                               # DestroyJob
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoce.so")
+
+NSYS = 12
+FP64, MIXED = 0, 1
+
+FIELDS = {
+    "sig": 0, "eps": 1, "edot": 2, "els": 3, "disgrad": 4, "crss": 5, "crss_bwd": 6,
+    "gamtot": 7, "gamdot": 8, "taux": 9, "tranmat": 10, "gamacum": 11, "svm": 12, "evm": 13,
+    "work_p": 14, "rho_rss": 15, "rho_crss": 16, "velgrad": 17,
+}
+
+
+class VoceError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = "voce error %d" % code
+        if _lib is not None:
+            msg += ": " + _lib.voce_strerror(code).decode()
+        if what:
+            msg = what + ": " + msg
+        super().__init__(msg)
+
+
+class Phase(C.Structure):
+    _fields_ = [("type", C.c_int), ("elast", C.c_double * 3), ("nsys", C.c_int),
+                ("bn", (C.c_double * 3) * NSYS), ("bb", (C.c_double * 3) * NSYS),
+                ("nsrs", C.c_double), ("gam0", C.c_double), ("tau0f", C.c_double),
+                ("tau0b", C.c_double), ("tau1", C.c_double), ("thet0", C.c_double),
+                ("thet1", C.c_double), ("hard", (C.c_double * NSYS) * NSYS)]
+
+
+class Setup(C.Structure):
+    _fields_ = [("dims", C.c_int * 3), ("n_phases", C.c_int), ("phase", Phase * 2),
+                ("time_step", C.c_double), ("err", C.c_double), ("iter_max", C.c_int),
+                ("update_flag", C.c_int), ("hard_flag", C.c_int),
+                ("velgrad_flag", C.c_int * 6), ("velgrad", C.c_double * 6),
+                ("stress_flag", C.c_int * 6), ("stress", C.c_double * 6),
+                ("use_full_velgrad", C.c_int), ("velgrad_full", C.c_double * 9),
+                ("euler_deg", C.POINTER(C.c_double)), ("grain", C.POINTER(C.c_int)),
+                ("phase_id", C.POINTER(C.c_int)), ("precision", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("istep", C.c_int), ("iter", C.c_int), ("iter_total", C.c_int),
+                ("e_vm", C.c_double), ("evmp", C.c_double), ("d_vm", C.c_double),
+                ("dvmp", C.c_double), ("s_vm", C.c_double), ("s_vm1", C.c_double),
+                ("sig_avg", C.c_double * 9), ("sig_avg1", C.c_double * 9),
+                ("disgrad_avg", C.c_double * 9), ("velgrad_avg", C.c_double * 9),
+                ("eps_avg", C.c_double * 9), ("edot_avg", C.c_double * 9),
+                ("err_e", C.POINTER(C.c_double)), ("err_s", C.POINTER(C.c_double)),
+                ("svm_it", C.POINTER(C.c_double))]
+
+
+class Case(C.Structure):
+    _fields_ = [("setup", Setup), ("n_steps", C.c_int), ("print_control", C.c_int * 2),
+                ("tex_flag", C.c_int), ("namelist_status", C.c_int * 26),
+                ("euler_deg", C.POINTER(C.c_double)), ("grain", C.POINTER(C.c_int)),
+                ("phase_id", C.POINTER(C.c_int))]
+
+
+# every symbol include/voce.h declares
+EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_field",
+           "voce_set_diagnostics", "voce_ctx_nvox", "voce_ctx_destroy", "voce_strerror",
+           "voce_load_case", "voce_case_free", "voce_write_fields",
+           # include/voce_prof.h
+           "voce_prof_enable", "voce_prof_read", "voce_ctx_info"]
+
+PROF_KERNELS = ["plane_fwd", "xgreen", "plane_inv", "nr", "fin_iter"]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libvoce.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise VoceError(-6, "native library %s not built (python -m voce_amd.build)" % path)
+    lib = C.CDLL(path)
+    lib.voce_strerror.restype = C.c_char_p
+    lib.voce_strerror.argtypes = [C.c_int]
+    lib.voce_field_ncomp.argtypes = [C.c_int]
+    lib.voce_ctx_create.argtypes = [C.POINTER(Setup), C.POINTER(C.c_void_p)]
+    lib.voce_increment.argtypes = [C.c_void_p, C.POINTER(Report)]
+    lib.voce_get_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+    lib.voce_set_diagnostics.argtypes = [C.c_void_p, C.c_int]
+    lib.voce_ctx_nvox.restype = C.c_long
+    lib.voce_ctx_nvox.argtypes = [C.c_void_p]
+    lib.voce_ctx_destroy.argtypes = [C.c_void_p]
+    lib.voce_ctx_destroy.restype = None
+    lib.voce_load_case.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Case)]
+    lib.voce_case_free.argtypes = [C.POINTER(Case)]
+    lib.voce_case_free.restype = None
+    lib.voce_write_fields.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int]
+    lib.voce_prof_enable.argtypes = [C.c_void_p, C.c_int]
+    lib.voce_prof_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    lib.voce_ctx_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+    _lib = lib
+    return lib
+
+
+class LoadedCase:
+    """A parsed input (namelist + .sx + mesh); owns the C arrays."""
+
+    def __init__(self, path: str, base_dir: str | None = None):
+        lib = load_library()
+        self._c = Case()
+        rc = lib.voce_load_case(path.encode(), base_dir.encode() if base_dir else None, C.byref(self._c))
+        if rc:
+            raise VoceError(rc, "voce_load_case(%s)" % path)
+        self._freed = False
+
+    @property
+    def setup(self) -> Setup:
+        return self._c.setup
+
+    @property
+    def n_steps(self) -> int:
+        return self._c.n_steps
+
+    @property
+    def namelist_status(self):
+        return list(self._c.namelist_status)
+
+    @property
+    def nvox(self):
+        d = self._c.setup.dims
+        return d[0] * d[1] * d[2]
+
+    def arrays(self):
+        V = self.nvox
+        e = np.ctypeslib.as_array(self._c.euler_deg, shape=(V * 3,)).reshape(V, 3).copy()
+        g = np.ctypeslib.as_array(self._c.grain, shape=(V,)).copy()
+        p = np.ctypeslib.as_array(self._c.phase_id, shape=(V,)).copy()
+        return e, g, p
+
+    def free(self):
+        if not self._freed:
+            load_library().voce_case_free(C.byref(self._c))
+            self._freed = True
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def load_case(path: str, base_dir: str | None = None) -> LoadedCase:
+    return LoadedCase(path, base_dir)
+
+
+def report_dict(r: Report) -> dict:
+    n = r.iter
+    d = {k: getattr(r, k) for k in ("istep", "iter", "iter_total", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")}
+    for k in ("sig_avg", "sig_avg1", "disgrad_avg", "velgrad_avg", "eps_avg", "edot_avg"):
+        d[k] = np.array(getattr(r, k)[:]).reshape(3, 3)
+    d["err_e"] = np.array(r.err_e[:n]) if n else np.zeros(0)
+    d["err_s"] = np.array(r.err_s[:n]) if n else np.zeros(0)
+    d["svm_it"] = np.array(r.svm_it[:n]) if n else np.zeros(0)
+    return d
+
+
+class Solver:
+    """One RVE resident on the GPU (a voce_ctx)."""
+
+    def __init__(self, setup: Setup, precision="fp64", keepalive=None):
+        lib = load_library()
+        setup.precision = MIXED if precision in ("mixed", MIXED) else FP64
+        self._keep = keepalive
+        h = C.c_void_p()
+        rc = lib.voce_ctx_create(C.byref(setup), C.byref(h))
+        if rc:
+            raise VoceError(rc, "voce_ctx_create")
+        self._h = h
+        self.nvox = lib.voce_ctx_nvox(h)
+
+    @classmethod
+    def from_case(cls, case: LoadedCase, precision="fp64"):
+        return cls(case.setup, precision, keepalive=case)
+
+    def increment(self) -> dict:
+        r = Report()
+        rc = load_library().voce_increment(self._h, C.byref(r))
+        if rc:
+            raise VoceError(rc, "voce_increment")
+        return report_dict(r)
+
+    def set_diagnostics(self, on=True):
+        rc = load_library().voce_set_diagnostics(self._h, int(bool(on)))
+        if rc:
+            raise VoceError(rc, "voce_set_diagnostics")
+
+    def field(self, name) -> np.ndarray:
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        lib = load_library()
+        nc = lib.voce_field_ncomp(fid)
+        out = np.empty((nc, self.nvox), dtype=np.float64)
+        rc = lib.voce_get_field(self._h, fid, out.ctypes.data_as(C.POINTER(C.c_double)), out.nbytes)
+        if rc:
+            raise VoceError(rc, "voce_get_field(%s)" % name)
+        return out
+
+    def write_fields(self, directory, step, tex_flag=0, final=False):
+        rc = load_library().voce_write_fields(self._h, directory.encode(), step, tex_flag, int(final))
+        if rc:
+            raise VoceError(rc, "voce_write_fields")
+
+    def prof_enable(self, on=True):
+        load_library().voce_prof_enable(self._h, int(bool(on)))
+
+    def prof_read(self) -> dict:
+        ms = (C.c_double * 5)()
+        n = (C.c_long * 5)()
+        load_library().voce_prof_read(self._h, ms, n)
+        return {k: (ms[i], n[i]) for i, k in enumerate(PROF_KERNELS)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().voce_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

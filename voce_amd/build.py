@@ -1,0 +1,49 @@
+"""Builds the native parts of voce_amd in-tree for gfx950.
+
+  voce_amd/libvoce.so   HIP kernels + host context + file formats (the C ABI
+                        of include/voce.h)
+  voce_amd/bin/voce     the command-line host driver
+
+Run ``python -m voce_amd.build``.  hipcc cross-compiles without a GPU.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+ARCH = os.environ.get("VOCE_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# Packed FP32 arithmetic is disabled for every HIP translation unit.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose_resources=False):
+    obj = os.path.join(HERE, "_obj")
+    os.makedirs(obj, exist_ok=True)
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    k_o = os.path.join(obj, "voce_kernels.o")
+    c_o = os.path.join(obj, "voce_ctx.o")
+    io_o = os.path.join(obj, "voce_io.o")
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "voce_kernels.hip"), "-o", k_o])
+    run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "voce_ctx.cpp"), "-o", c_o])
+    run(["gcc", "-O2", "-fPIC", "-std=gnu99", "-Wall", "-c", os.path.join(CSRC, "voce_io.c"), "-o", io_o])
+    lib = os.path.join(HERE, "libvoce.so")
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, k_o, c_o, io_o, "-o", lib])
+    run(["gcc", "-O2", "-std=gnu99", "-Wall", os.path.join(CSRC, "voce_main.c"), "-o",
+         os.path.join(HERE, "bin", "voce"), "-L" + HERE, "-lvoce", "-Wl,-rpath,$ORIGIN/.."])
+    return lib
+
+
+if __name__ == "__main__":
+    build(verbose_resources="-v" in sys.argv)

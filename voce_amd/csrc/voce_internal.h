@@ -1,0 +1,105 @@
+// voce_internal.h -- device data layout and kernel launchers shared by the
+// host context code (voce_ctx.cpp) and the HIP kernels (voce_kernels.hip).
+//
+// Layout in HBM for one RVE with V = Nx*Ny*Nz voxels, voxel index
+// v = (x*Ny + y)*Nz + z (the reference's order, evp.h:32-33).  Every field
+// is structure-of-arrays, component-major (component c at base + c*V):
+//
+//   E6     6 x V f64   sym(DisGrad) in the Lebensohn-Tome B basis
+//   S6     6 x V f64   Cauchy stress, B basis
+//   P6     6 x V f64   stress at which the last NR flow-rule evaluation ran
+//                      (the "pre-update" stress; Edot and gamdot follow from
+//                      it exactly, evolution.c:67,125-133)
+//   EP5    5 x V f64   plastic strain, B basis (the 6th component is 0)
+//   ED5    5 x V f64   plastic strain rate of the increment, B basis
+//   CRSS  12 x V f64   forward CRSS; a second 12 x V block holds the
+//                      backward CRSS only when tau0f != tau0b
+//   R      9 x V f64   TranMat_xt2sa (crystal -> sample), row-major
+//   WG     3 x V f64   skew(DisGrad): components (2,1), (0,2), (1,0)
+//   WF     3 x V f64   skew(DisGrad increment) of the last increment (its
+//                      Green-operator part until the close-out completes it)
+//   GAC      V f64     accumulated shear gamacum;  GTOT 12 x V gammatotal
+//   ORI      V u32     index of the voxel's initial orientation in the
+//                      per-orientation compliance table (0xffffffff = gas)
+//   PH       V u8      phase id (1-based)
+//
+// FFT work buffers (half spectrum along z, NZH = Nz/2 + 1, complex of the
+// FFT type T: double2 in FP64 mode, float2 in MIXED mode):
+//   SPEC  6 x Nx x Ny x NZH   index ((c*Nx + kx)*Ny + ky)*NZH + kz
+//   USUM  3 x Nx x Ny x NZH   sum over the increment's iterations of the
+//                             Green displacement u_hat (skew recovery)
+#pragma once
+#include <stdint.h>
+
+#define VOCE_MAXIT_LOG 4096
+#define VOCE_GAS 0xffffffffu
+
+struct PhaseConst {
+    int nsys;
+    int gas;
+    int two_crss;   // 1 when forward and backward CRSS are stored separately
+    int int_n;      // 1 when nsrs is an integer in [1, 64]
+    double nsr, gam0, tau0f, tau0b, tau1, thet0, thet1;
+    double P3[12][9];      // crystal-frame Schmid tensors (chg_basis5 opt 1 of Schm_xt)
+    double bn[12][3], bb[12][3];
+    double hard[12][12];
+};
+
+// Macro (RVE-wide) state, resident in device memory.  Updated by the
+// single-block finaliser kernels; read with uniform loads by the rest.
+struct MacroState {
+    // constants
+    double dt, err_tol, wgt, wgt_over_ph1;   // wgt_over_ph1 = WGT/Wgt_ph1
+    double C066[36];
+    double S0[81];
+    double C0[81];
+    double Udot[9], Udot_a[9], Scauchy[9];
+    double udot_sym6[6];    // B basis of sym(Udot)
+    int velgrad_flag[6], stress_flag[6];
+    int iter_max, creep, update_flag, hard_flag;
+    int n_phases, diag, two_crss_any, pad0;
+    // evolving
+    double DisGradAvg[9], DisGradAvg_t[9], DisGradAvg_actual[9];
+    double dD[9], acum[9], applied[9];
+    double SigAvg[9], SigAvg1[9];
+    double EpsAvg[9], EdotAvg[9], VelGradAvg[9];
+    double e_vm, d_vm, s_vm, s_vm1, Err_e, Err_s, evmp, dvmp;
+    double shift6[6];        // uniform B-basis strain added by the next E update
+    double creep_sym6[6];    // creep mode: uniform sym(DisGrad) at predictor
+    double creep_skew[3];    // creep mode: uniform skew(DisGrad) at predictor
+    int istep, iter, iter_total, active;
+    int error, pad1, pad2, pad3;
+    double log_err_e[VOCE_MAXIT_LOG], log_err_s[VOCE_MAXIT_LOG], log_svm[VOCE_MAXIT_LOG];
+};
+
+// Device pointers of one RVE.
+struct RveDev {
+    int nx, ny, nz, nzh;
+    long V, nspec;               // voxels, complex points of one half-spectrum component
+    double *E6, *S6, *P6, *EP5, *ED5, *CRSS, *CRSSB, *R, *WG, *WF, *GAC, *GTOT;
+    double *GAMDOT, *TAUX, *SVM, *EVM, *WORKP, *RHOR, *RHOC;   // diagnostics (may be null)
+    uint32_t *ORI;
+    uint8_t *PH;
+    double *SLOC;                // per orientation: S_local = C_gr^-1, 36 doubles
+    PhaseConst *phase;           // [2]
+    MacroState *ms;
+    void *spec, *usum;           // complex T
+    double *part;                // per-block partials
+    int nblk_vox;                // blocks of the voxel kernels
+    void *tw_x, *tw_y, *tw_z;    // twiddle tables, complex T, length N, exp(-2 pi i j/N)
+    int fft_f32;                 // 1: FFT in FP32
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// host-side launchers (voce_kernels.hip); each returns a hipError_t as int
+int voce_launch_begin_increment(const RveDev *r, void *stream);
+int voce_launch_iteration(const RveDev *r, void *stream, void *events /* hipEvent_t[6] or NULL */);
+int voce_launch_closeout(const RveDev *r, void *stream, int need_skew);
+int voce_launch_init_fields(const RveDev *r, const double *sig6_by_orient, void *stream);
+int voce_grid_supported(int n);
+int voce_kernels_prepare(void);
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,1410 @@
+// voce_kernels.hip -- CDNA4 (gfx950) kernels of one EVP-FFT increment.
+//
+// One FFT fixed-point iteration (evolution.c:342-431) is five launches:
+//   k_plane_fwd  per (x-plane, stress component): real-to-half-complex FFT
+//                along z (two real rows packed per complex row) and complex
+//                FFT along y, all in LDS.                       (evolution.c:350-361)
+//   k_xgreen     per tile of (y, kz) columns, all 6 components: FFT along
+//                x, Green operator applied on the fly from n(k) and C0
+//                (init.c:570-615, evolution.c:363-377), inverse FFT along x.
+//   k_plane_inv  per (x-plane, component): inverse y FFT, half-complex-to-
+//                real z FFT, and the strain update
+//                E += dDisGradAvg + IFFT/N (evolution.c:383-394).
+//   k_nr         one voxel per lane: the 6x6 Newton-Raphson of UpdateStress
+//                (evolution.c:3-168) with StrainRate_eval (constitutive.c:4-72)
+//                and the LU of LU_inv_66 (kinematics.c:282-424) in registers,
+//                plus wavefront-shuffle partial sums of the error norms and
+//                the stress averages.
+//   k_fin_iter   one block: fixed-order sum of the partials, get_smacro
+//                (evolution.c:170-241), error normalisation and the
+//                convergence test (evolution.c:342, 405-428).
+// Every launch reads MacroState::active first, so iterations enqueued past
+// convergence return at once and the host never blocks inside a loop.
+//
+// The increment close-out (evolution.c:433-530) is k_xgreen/k_plane_inv in
+// skew mode (lattice spin from the summed Green displacements), k_closeout
+// (plastic strain, update_orient, harden, diagnostics) and k_fin_incr.
+//
+// FFT type T is double (FP64 mode) or float (MIXED mode); the constitutive
+// update, the reductions and all state stay double.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "voce_internal.h"
+
+namespace {
+
+constexpr double RSQ2 = 0.70710678118654744;   // kinematics.c:22-24
+constexpr double RSQ3 = 0.57735026918962584;
+constexpr double RSQ6 = 0.40824829046386304;
+constexpr double TWO_PI_REF = 6.28318530718;    // evp.h:58
+
+template <typename T> struct cplx_of;
+template <> struct cplx_of<double> { using t = double2; };
+template <> struct cplx_of<float> { using t = float2; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C cmul_mi(C a)
+{
+    return INV ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// ------------------------------------------------------------------
+// Radix butterflies (forward sign -1, inverse +1)
+// ------------------------------------------------------------------
+template <int R, bool INV, typename C> struct Bfly;
+
+template <bool INV, typename C> struct Bfly<2, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV, typename C> struct Bfly<3, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        using T = decltype(v[0].x);
+        const T c1 = (T)-0.5;
+        const T s1 = INV ? (T)0.86602540378443865 : (T)-0.86602540378443865;
+        C a = v[0], b = v[1], c = v[2];
+        C t = cadd(b, c);
+        C d = csub(b, c);
+        v[0] = cadd(a, t);
+        C m{a.x + c1 * t.x, a.y + c1 * t.y};
+        C r{-s1 * d.y, s1 * d.x};   // i*s1*d
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    }
+};
+
+template <bool INV, typename C> struct Bfly<4, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        C a0 = cadd(v[0], v[2]);
+        C a1 = csub(v[0], v[2]);
+        C a2 = cadd(v[1], v[3]);
+        C a3 = cmul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(a0, a2);
+        v[2] = csub(a0, a2);
+        v[1] = cadd(a1, a3);
+        v[3] = csub(a1, a3);
+    }
+};
+
+template <bool INV, typename C> struct Bfly<8, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        using T = decltype(v[0].x);
+        const T h = (T)0.70710678118654752;
+        // radix-2 on pairs (k, k+4), twiddle, two radix-4 on evens/odds
+        C e[4], o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            e[k] = cadd(v[k], v[k + 4]);
+            o[k] = csub(v[k], v[k + 4]);
+        }
+        // o[k] *= W8^k
+        {
+            C t1 = o[1];
+            o[1] = INV ? C{h * (t1.x - t1.y), h * (t1.x + t1.y)} : C{h * (t1.x + t1.y), h * (t1.y - t1.x)};
+            o[2] = cmul_mi<INV>(o[2]);
+            C t3 = o[3];
+            o[3] = INV ? C{-h * (t3.x + t3.y), h * (t3.x - t3.y)} : C{h * (t3.y - t3.x), -h * (t3.x + t3.y)};
+        }
+        Bfly<4, INV, C>::run(e);
+        Bfly<4, INV, C>::run(o);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[2 * k] = e[k];
+            v[2 * k + 1] = o[k];
+        }
+    }
+};
+
+// ------------------------------------------------------------------
+// One Stockham stage over `nlines` lines of length N held in LDS.
+// Element q of line l lives at s[l*lstride + q*estride].  All loads of a
+// stage complete (barrier) before any store, so the stage runs in place.
+// LF: consecutive lanes take consecutive lines (for strided lines).
+// ------------------------------------------------------------------
+template <typename C, int R, int NT, int MAXEL, bool INV, bool LF>
+__device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int lstride, int estride,
+                                          const C *tw)
+{
+    constexpr int BPT = (MAXEL / R + NT - 1) / NT;
+    const int nbl = N / R;
+    const int nb = nlines * nbl;
+    C v[BPT][R];
+    int base[BPT], jj[BPT];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int b = (int)threadIdx.x + i * NT;
+        if (b < nb) {
+            int l, j;
+            if (LF) { j = b / nlines; l = b - j * nlines; }
+            else { l = b / nbl; j = b - l * nbl; }
+            base[i] = l * lstride;
+            jj[i] = j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = s[base[i] + (j + r * nbl) * estride];
+        }
+    }
+    __syncthreads();
+    const int twstep = N / (Ns * R);
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int b = (int)threadIdx.x + i * NT;
+        if (b < nb) {
+            const int j = jj[i];
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C w = tw[r * k * twstep];
+                if (INV) w.y = -w.y;
+                v[i][r] = cmul(v[i][r], w);
+            }
+            Bfly<R, INV, C>::run(v[i]);
+            const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[base[i] + (d + r * Ns) * estride] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Mixed-radix plan: radix-3 stages, then radix 8/4/2 for the power of two.
+template <typename C, int NT, int MAXEL, bool INV, bool LF>
+__device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, const C *tw)
+{
+    int Ns = 1, rem = N;
+    while (rem % 3 == 0) {
+        fft_stage<C, 3, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw);
+        Ns *= 3;
+        rem /= 3;
+    }
+    int a = 0;
+    while ((1 << a) < rem) ++a;
+    int n8 = a / 3, n4 = 0, n2 = 0;
+    if (a % 3 == 2) n4 = 1;
+    if (a % 3 == 1) {
+        if (n8 >= 1) { n8 -= 1; n4 = 2; }
+        else n2 = 1;
+    }
+    for (int i = 0; i < n8; ++i) { fft_stage<C, 8, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 8; }
+    for (int i = 0; i < n4; ++i) { fft_stage<C, 4, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 4; }
+    for (int i = 0; i < n2; ++i) { fft_stage<C, 2, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 2; }
+}
+
+__device__ __forceinline__ bool iter_active(const MacroState *ms)
+{
+    return ms->active != 0;
+}
+
+// ------------------------------------------------------------------
+// Forward plane kernel: S6 component c of plane x -> SPEC[c][x][:][:]
+// grid (nx, 6), dynamic LDS (ny*nzh + ny + nz) complex
+// ------------------------------------------------------------------
+template <typename T, int NT, int MAXEL>
+__global__ __launch_bounds__(NT) void k_plane_fwd(RveDev r)
+{
+    using C = typename cplx_of<T>::t;
+    if (!iter_active(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
+    C *twy = s + ny * nzh;
+    C *twz = twy + ny;
+    const int x = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const C *gty = (const C *)r.tw_y;
+    const C *gtz = (const C *)r.tw_z;
+    for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
+    for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
+    const double *src = r.S6 + (long)c * r.V + (long)x * ny * nz;
+    T *sr = (T *)s;
+    for (int e = tid; e < ny * nz; e += NT) {
+        const int y = e / nz, z = e - y * nz;
+        sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)] = (T)src[e];
+    }
+    __syncthreads();
+    fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, 2 * nzh, 1, twz);
+    // split packed rows a + i b into the half spectra of a and b
+    {
+        constexpr int SPT = (MAXEL / 2 + NT - 1) / NT;
+        C A[SPT], B[SPT];
+        const int nit = (ny / 2) * nzh;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT;
+            if (it < nit) {
+                const int p = it / nzh, k = it - p * nzh;
+                const C z0 = s[p * 2 * nzh + k];
+                const int km = (k == 0) ? 0 : nz - k;
+                const C z1 = s[p * 2 * nzh + km];
+                A[i] = C{(T)0.5 * (z0.x + z1.x), (T)0.5 * (z0.y - z1.y)};
+                B[i] = C{(T)0.5 * (z0.y + z1.y), (T)0.5 * (z1.x - z0.x)};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT;
+            if (it < nit) {
+                const int p = it / nzh, k = it - p * nzh;
+                s[(2 * p) * nzh + k] = A[i];
+                s[(2 * p + 1) * nzh + k] = B[i];
+            }
+        }
+        __syncthreads();
+    }
+    fft_lines<C, NT, MAXEL, false, true>(s, nzh, ny, 1, nzh, twy);
+    C *dst = (C *)r.spec + ((long)c * nx + x) * ny * nzh;
+    for (int e = tid; e < ny * nzh; e += NT) dst[e] = s[e];
+}
+
+// ------------------------------------------------------------------
+// Inverse plane kernel.  mode 0: E6[c] += shift[c] + result (or, in the
+// first iteration of a creep increment, E6[c] = creep[c] + result);
+// mode 1: WF[c] = result (skew recovery).  grid (nx, ncomp)
+// ------------------------------------------------------------------
+template <typename T, int NT, int MAXEL>
+__global__ __launch_bounds__(NT) void k_plane_inv(RveDev r, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    if (mode == 0 && !iter_active(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
+    C *twy = s + ny * nzh;
+    C *twz = twy + ny;
+    const int x = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const C *gty = (const C *)r.tw_y;
+    const C *gtz = (const C *)r.tw_z;
+    for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
+    for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
+    const C *srcp = (const C *)r.spec + ((long)c * nx + x) * ny * nzh;
+    for (int e = tid; e < ny * nzh; e += NT) s[e] = srcp[e];
+    __syncthreads();
+    fft_lines<C, NT, MAXEL, true, true>(s, nzh, ny, 1, nzh, twy);
+    // pack the half spectra of rows 2p (a) and 2p+1 (b) into a full complex
+    // row Z = A + iB, A and B Hermitian-extended
+    {
+        constexpr int SPT = (MAXEL + NT - 1) / NT;
+        C Z[SPT];
+        const int nit = (ny / 2) * nz;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT;
+            if (it < nit) {
+                const int p = it / nz, k = it - p * nz;
+                C A, B;
+                if (k <= nz / 2) {
+                    A = s[(2 * p) * nzh + k];
+                    B = s[(2 * p + 1) * nzh + k];
+                    if (k == 0 || 2 * k == nz) { A.y = (T)0; B.y = (T)0; }
+                } else {
+                    A = s[(2 * p) * nzh + nz - k];
+                    B = s[(2 * p + 1) * nzh + nz - k];
+                    A.y = -A.y;
+                    B.y = -B.y;
+                }
+                Z[i] = C{A.x - B.y, A.y + B.x};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT;
+            if (it < nit) {
+                const int p = it / nz, k = it - p * nz;
+                s[p * 2 * nzh + k] = Z[i];
+            }
+        }
+        __syncthreads();
+    }
+    fft_lines<C, NT, MAXEL, true, false>(s, ny / 2, nz, 2 * nzh, 1, twz);
+    const T *sr = (const T *)s;
+    const long off = (long)c * r.V + (long)x * ny * nz;
+    if (mode == 0) {
+        const MacroState *ms = r.ms;
+        double *E = r.E6 + off;
+        const bool creep_set = (ms->creep != 0) && (ms->iter == 0);
+        const double sh = creep_set ? 0.0 : ms->shift6[c];
+        const double cr = ms->creep_sym6[c];
+        for (int e = tid; e < ny * nz; e += NT) {
+            const int y = e / nz, z = e - y * nz;
+            const double val = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
+            if (creep_set) E[e] = cr + (sh + val);
+            else E[e] = E[e] + (sh + val);
+        }
+    } else {
+        double *W = r.WF + off;
+        for (int e = tid; e < ny * nz; e += NT) {
+            const int y = e / nz, z = e - y * nz;
+            W[e] = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
+        }
+    }
+}
+
+// unit frequency vector, init.c:660-677
+template <typename T>
+__device__ __forceinline__ void kvec(int kx, int ky, int kz, int nx, int ny, int nz, T n[3])
+{
+    double t0 = (double)kx * (double)(1.0 / nx);
+    double t1 = (double)ky * (double)(1.0 / ny);
+    double t2 = (double)kz * (double)(1.0 / nz);
+    t0 = (t0 - round(t0)) * TWO_PI_REF;
+    t1 = (t1 - round(t1)) * TWO_PI_REF;
+    t2 = (t2 - round(t2)) * TWO_PI_REF;
+    const double g = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    const double ig = 1. / g;
+    n[0] = (T)(t0 * ig);
+    n[1] = (T)(t1 * ig);
+    n[2] = (T)(t2 * ig);
+}
+
+// ------------------------------------------------------------------
+// x-direction FFT + Green operator.  grid ceil(ny*nzh / tk)
+// mode 0 (iteration): SPEC (6 comps) -> fwd x FFT -> Delta = B(sym(u (x) n))
+//   with u = -omega(n) sigma_hat n / V -> inverse x FFT -> SPEC; USUM += u.
+// mode 1 (skew): USUM (3 comps) -> skew(u (x) n) -> inverse x FFT -> SPEC[0..2]
+// ------------------------------------------------------------------
+template <typename T, int NT, int MAXEL>
+__global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
+{
+    using C = typename cplx_of<T>::t;
+    const MacroState *ms = r.ms;
+    if (mode == 0 && !iter_active(ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
+    const int ncolt = ny * nzh;
+    C *twx = s + 6 * tk * nx;
+    const int tid = threadIdx.x;
+    const C *gtx = (const C *)r.tw_x;
+    for (int i = tid; i < nx; i += NT) twx[i] = gtx[i];
+    const int col0 = blockIdx.x * tk;
+    const int ncol = min(tk, ncolt - col0);
+    const int ncomp = (mode == 0) ? 6 : 3;
+    const C *src = (const C *)((mode == 0) ? r.spec : r.usum);
+    const long cstride = (long)nx * ncolt;
+    for (int e = tid; e < ncomp * nx * ncol; e += NT) {
+        const int cc = e / (nx * ncol);
+        const int rem = e - cc * nx * ncol;
+        const int x = rem / ncol, t = rem - x * ncol;
+        s[(cc * tk + t) * nx + x] = src[cc * cstride + (long)x * ncolt + col0 + t];
+    }
+    __syncthreads();
+    if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, nx, 1, twx);
+    const bool first = (ms->iter == 0);
+    C *usum = (C *)r.usum;
+    const T invV = (T)(1.0 / (double)r.V);
+    for (int e = tid; e < nx * ncol; e += NT) {
+        const int t = e / nx, x = e - t * nx;
+        const int col = col0 + t;
+        const int ky = col / nzh, kz = col - ky * nzh;
+        T n[3];
+        const bool zero = (x == 0 && ky == 0 && kz == 0);
+        if (!zero) kvec<T>(x, ky, kz, nx, ny, nz, n);
+        else n[0] = n[1] = n[2] = (T)0;
+        const long gidx = (long)x * ncolt + col;
+        if (mode == 0) {
+            C a[6];
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) a[cc] = s[(cc * tk + t) * nx + x];
+            C u[3] = {C{0, 0}, C{0, 0}, C{0, 0}};
+            if (!zero) {
+                // omega^-1_ik = C0_ijkl n_j n_l, init.c:595-602
+                T K[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        T acc = 0;
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+#pragma unroll
+                            for (int l = 0; l < 3; ++l) acc += (T)ms->C0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
+                        K[i][k] = acc;
+                    }
+                // M3Inverse, init.c:546-568
+                const T det = K[0][0] * (K[1][1] * K[2][2] - K[1][2] * K[2][1]) -
+                              K[0][1] * (K[1][0] * K[2][2] - K[1][2] * K[2][0]) +
+                              K[0][2] * (K[1][0] * K[2][1] - K[2][0] * K[1][1]);
+                const T id = (T)1 / det;
+                T om[3][3];
+                om[0][0] = (K[1][1] * K[2][2] - K[1][2] * K[2][1]) * id;
+                om[0][1] = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) * id;
+                om[0][2] = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) * id;
+                om[1][0] = (K[1][2] * K[2][0] - K[1][0] * K[2][2]) * id;
+                om[1][1] = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) * id;
+                om[1][2] = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) * id;
+                om[2][0] = (K[1][0] * K[2][1] - K[1][1] * K[2][0]) * id;
+                om[2][1] = (K[0][1] * K[2][0] - K[0][0] * K[2][1]) * id;
+                om[2][2] = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) * id;
+                // sigma_hat tensor from its B-basis components
+                const T q2 = (T)RSQ2, q3 = (T)RSQ3, q6 = (T)RSQ6;
+                C sg[3][3];
+                sg[0][0] = C{-q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, -q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
+                sg[1][1] = C{q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
+                sg[2][2] = C{(T)2 * q6 * a[1].x + q3 * a[5].x, (T)2 * q6 * a[1].y + q3 * a[5].y};
+                sg[1][2] = sg[2][1] = C{q2 * a[2].x, q2 * a[2].y};
+                sg[0][2] = sg[2][0] = C{q2 * a[3].x, q2 * a[3].y};
+                sg[0][1] = sg[1][0] = C{q2 * a[4].x, q2 * a[4].y};
+                C tv[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    tv[i] = C{(sg[i][0].x * n[0] + sg[i][1].x * n[1] + sg[i][2].x * n[2]) * invV,
+                              (sg[i][0].y * n[0] + sg[i][1].y * n[1] + sg[i][2].y * n[2]) * invV};
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    u[i] = C{-(om[i][0] * tv[0].x + om[i][1] * tv[1].x + om[i][2] * tv[2].x),
+                             -(om[i][0] * tv[0].y + om[i][1] * tv[1].y + om[i][2] * tv[2].y)};
+            }
+            // Delta grad u = u_i n_j; its symmetric part in the B basis
+            C d[6];
+#define UN(i, j) C{u[i].x * n[j], u[i].y * n[j]}
+            {
+                const T q2 = (T)RSQ2, q3 = (T)RSQ3, q6 = (T)RSQ6;
+                C u00 = UN(0, 0), u11 = UN(1, 1), u22 = UN(2, 2);
+                C u12 = UN(1, 2), u21 = UN(2, 1), u02 = UN(0, 2), u20 = UN(2, 0), u01 = UN(0, 1), u10 = UN(1, 0);
+                d[0] = C{q2 * (u11.x - u00.x), q2 * (u11.y - u00.y)};
+                d[1] = C{q6 * ((T)2 * u22.x - u00.x - u11.x), q6 * ((T)2 * u22.y - u00.y - u11.y)};
+                d[2] = C{q2 * (u12.x + u21.x), q2 * (u12.y + u21.y)};
+                d[3] = C{q2 * (u02.x + u20.x), q2 * (u02.y + u20.y)};
+                d[4] = C{q2 * (u01.x + u10.x), q2 * (u01.y + u10.y)};
+                d[5] = C{q3 * (u00.x + u11.x + u22.x), q3 * (u00.y + u11.y + u22.y)};
+            }
+#undef UN
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) s[(cc * tk + t) * nx + x] = d[cc];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                C *up = usum + i * cstride + gidx;
+                *up = first ? u[i] : cadd(*up, u[i]);
+            }
+        } else {
+            C u[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) u[i] = s[(i * tk + t) * nx + x];
+            // skew(u (x) n): components (2,1), (0,2), (1,0)
+            C w0 = C{(T)0.5 * (u[2].x * n[1] - u[1].x * n[2]), (T)0.5 * (u[2].y * n[1] - u[1].y * n[2])};
+            C w1 = C{(T)0.5 * (u[0].x * n[2] - u[2].x * n[0]), (T)0.5 * (u[0].y * n[2] - u[2].y * n[0])};
+            C w2 = C{(T)0.5 * (u[1].x * n[0] - u[0].x * n[1]), (T)0.5 * (u[1].y * n[0] - u[0].y * n[1])};
+            s[(0 * tk + t) * nx + x] = w0;
+            s[(1 * tk + t) * nx + x] = w1;
+            s[(2 * tk + t) * nx + x] = w2;
+        }
+    }
+    __syncthreads();
+    fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, nx, 1, twx);
+    C *dst = (C *)r.spec;
+    for (int e = tid; e < ncomp * nx * ncol; e += NT) {
+        const int cc = e / (nx * ncol);
+        const int rem = e - cc * nx * ncol;
+        const int x = rem / ncol, t = rem - x * ncol;
+        dst[cc * cstride + (long)x * ncolt + col0 + t] = s[(cc * tk + t) * nx + x];
+    }
+}
+
+// ------------------------------------------------------------------
+// Constitutive pieces (double, registers)
+// ------------------------------------------------------------------
+
+// Schm_gr of update_schmid (kinematics.c:426-459): P = R P_xtal R^T,
+// projected on b1..b5.
+__device__ __forceinline__ void schmid_sample(const double R[9], const PhaseConst *pc, int nsys,
+                                              double P[12][5])
+{
+#pragma unroll
+    for (int s = 0; s < 12; ++s) {
+        if (s < nsys) {
+            const double *a = pc->P3[s];
+            double M[9], Q[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    M[i * 3 + j] = R[i * 3 + 0] * a[0 * 3 + j] + R[i * 3 + 1] * a[1 * 3 + j] + R[i * 3 + 2] * a[2 * 3 + j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    Q[i * 3 + j] = M[i * 3 + 0] * R[j * 3 + 0] + M[i * 3 + 1] * R[j * 3 + 1] + M[i * 3 + 2] * R[j * 3 + 2];
+            P[s][0] = Q[0] * (-RSQ2) + Q[4] * RSQ2;
+            P[s][1] = Q[0] * (-1. * RSQ6) + Q[4] * (-1. * RSQ6) + Q[8] * (2. * RSQ6);
+            P[s][2] = Q[5] * RSQ2 + Q[7] * RSQ2;
+            P[s][3] = Q[2] * RSQ2 + Q[6] * RSQ2;
+            P[s][4] = Q[1] * RSQ2 + Q[3] * RSQ2;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) P[s][k] = 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ double ipow(double x, int n)
+{
+    double r = 1.0;
+    while (n) {
+        if (n & 1) r *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return r;
+}
+
+// StrainRate_eval, constitutive.c:4-72.  D holds the symmetric 5x5
+// d(edot)/d(sigma) packed row-major upper (15 entries).
+struct FlowOut {
+    double edot[5];
+    double D[15];
+};
+
+__device__ __forceinline__ void flow_rule(const double sig[6], const double P[12][5], const double *crf,
+                                          const double *crb, long V, long v, const PhaseConst *pc, int nsys,
+                                          FlowOut &o, double *gd, double *tx)
+{
+#pragma unroll
+    for (int i = 0; i < 5; ++i) o.edot[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 15; ++i) o.D[i] = 0.0;
+    const double nsr = pc->nsr, g0 = pc->gam0;
+    const int intn = pc->int_n;
+    const int ni = (int)nsr;
+#pragma unroll
+    for (int s = 0; s < 12; ++s) {
+        if (s < nsys) {
+            const double rss0 = P[s][0] * sig[0] + P[s][1] * sig[1] + P[s][2] * sig[2] + P[s][3] * sig[3] +
+                                P[s][4] * sig[4];
+            const double tc = (rss0 < 0) ? crb[s * V + v] : crf[s * V + v];
+            const double rr = rss0 / tc;
+            double pn1, pn;
+            if (intn) {
+                const double ar = fabs(rr);
+                pn1 = ipow(ar, ni - 1);
+                pn = pn1 * ar;
+            } else {
+                pn1 = fabs(pow(rr, nsr - 1.));
+                pn = fabs(pow(rr, nsr));
+            }
+            const double rss1 = g0 * nsr * pn1 / tc;
+            const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+            if (gd) gd[s] = rss2;
+            if (tx) tx[s] = rss1;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) o.edot[i] += P[s][i] * rss2;
+            int q = 0;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+#pragma unroll
+                for (int j = i; j < 5; ++j) { o.D[q] += P[s][i] * P[s][j] * rss1; ++q; }
+        }
+    }
+}
+
+__device__ __forceinline__ double Dget(const double D[15], int i, int j)
+{
+    // compile-time i, j after unrolling
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    const int q = a * 5 - (a * (a - 1)) / 2 + (b - a);
+    return D[q];
+}
+
+// Crout LU with implicit row scaling and partial pivoting
+// (LU_dcmp, kinematics.c:282-354) applied to J x = b; b is permuted along.
+// Returns 0, or 35 for a row whose largest entry is below 1e-5.
+__device__ __forceinline__ int lu_solve6(double a[6][6], double b[6])
+{
+    double vv[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double big = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) big = fmax(big, fabs(a[i][j]));
+        if (big < 1E-5) return 35;
+        vv[i] = 1.0 / big;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+        for (int i = 0; i < j; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+        }
+        double big = 0.0;
+        int imax = j;
+#pragma unroll
+        for (int i = j; i < 6; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+            const double dum = vv[i] * fabs(sum);
+            if (dum >= big) { big = dum; imax = i; }
+        }
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            if (imax == i) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const double t = a[i][k];
+                    a[i][k] = a[j][k];
+                    a[j][k] = t;
+                }
+                vv[i] = vv[j];
+                const double tb = b[i];
+                b[i] = b[j];
+                b[j] = tb;
+            }
+        }
+        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
+        if (j != 5) {
+            const double dum = 1.0 / a[j][j];
+#pragma unroll
+            for (int i = j + 1; i < 6; ++i) a[i][j] *= dum;
+        }
+    }
+    // forward (unit lower) and back substitution
+#pragma unroll
+    for (int i = 1; i < 6; ++i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum / a[i][i];
+    }
+    return 0;
+}
+
+template <int NT, int NV>
+__device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *out)
+{
+    __shared__ double red[NT / 64][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double x = vals[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        vals[i] = x;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[w][i] = vals[i];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
+        out[threadIdx.x] = acc;
+    }
+}
+
+// ------------------------------------------------------------------
+// Newton-Raphson stress update, UpdateStress (evolution.c:3-168).
+// Partials per block: [0] sum|sigma-lambda|, [1] sum|eps_tot-eps|,
+// [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1.
+// ------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_nr(RveDev r)
+{
+    const MacroState *ms = r.ms;
+    if (!iter_active(ms)) return;
+    const long V = r.V;
+    const long v = (long)blockIdx.x * NT + threadIdx.x;
+    double acc[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    if (v < V) {
+        const uint32_t ori = r.ORI[v];
+        const int ph = r.PH[v];
+        if (ori == VOCE_GAS) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                r.S6[i * V + v] = 0.0;
+                r.P6[i * V + v] = 0.0;
+            }
+        } else {
+            const PhaseConst *pc = r.phase + (ph - 1);
+            const int nsys = pc->nsys;
+            const double dt = ms->dt;
+            double strain6[6], lam[6], eps6[6], sig[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                strain6[i] = r.E6[i * V + v];
+                lam[i] = r.S6[i * V + v];
+                sig[i] = lam[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i) eps6[i] = r.EP5[i * V + v];
+            eps6[5] = 0.0;
+            double Rm[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+            double P[12][5];
+            schmid_sample(Rm, pc, nsys, P);
+            const double *crf = r.CRSS;
+            const double *crb = pc->two_crss ? r.CRSSB : r.CRSS;
+            const double *Sl = r.SLOC + (long)ori * 36;
+            const double *C0 = ms->C066;
+            double signorm = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) signorm += lam[i] * lam[i];
+            signorm = sqrt(signorm);
+            double erral = 1E-6;   // 10 * erroral
+            double conv_e = 0.0, conv_s = 0.0;
+            double sig_old[6];
+            int iterl = 0;
+            while ((iterl < 100) && (fabs(erral) > 1E-7)) {
+                ++iterl;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) sig_old[i] = sig[i];
+                FlowOut fo;
+                flow_rule(sig, P, crf, crb, V, v, pc, nsys, fo, nullptr, nullptr);
+                double tot[6], res[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double tacc = eps6[i] + (i < 5 ? fo.edot[i] : 0.0) * dt;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) tacc += Sl[i * 6 + j] * sig[j];
+                    tot[i] = tacc;
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double racc = sig[i] - lam[i];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) racc += C0[i * 6 + j] * (tot[j] - strain6[j]);
+                    res[i] = racc;
+                }
+                double J[6][6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        double jacc = (double)(i == j);
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            const double dk = (k < 5 && j < 5) ? Dget(fo.D, k, j) : 0.0;
+                            jacc += C0[i * 6 + k] * (Sl[k * 6 + j] + dk * dt);
+                        }
+                        J[i][j] = jacc;
+                    }
+                if (lu_solve6(J, res)) {
+                    r.ms->error = 35;
+                    break;
+                }
+                double cn = 0.0, ce = 0.0, cs = 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    sig[i] -= res[i];
+                    cn += (sig[i] - sig_old[i]) * (sig[i] - sig_old[i]);
+                    ce += (tot[i] - strain6[i]) * (tot[i] - strain6[i]);
+                    cs += (sig[i] - lam[i]) * (sig[i] - lam[i]);
+                }
+                erral = sqrt(cn) / signorm;
+                conv_e = sqrt(ce);
+                conv_s = sqrt(cs);
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                r.S6[i * V + v] = sig[i];
+                r.P6[i * V + v] = sig_old[i];
+            }
+            acc[0] = conv_s;
+            acc[1] = conv_e;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                acc[2 + i] = sig[i];
+                acc[8 + i] = (ph == 1) ? sig[i] : 0.0;
+            }
+        }
+    }
+    block_reduce_store<NT, 14>(acc, r.part + (long)blockIdx.x * 16);
+}
+
+// tensor of a B-basis 6-vector (chg_basis opt 1)
+__device__ __forceinline__ void b6_to_t(const double a[6], double T[9])
+{
+    T[0] = -RSQ2 * a[0] - RSQ6 * a[1] + RSQ3 * a[5];
+    T[4] = RSQ2 * a[0] - RSQ6 * a[1] + RSQ3 * a[5];
+    T[8] = 2. * RSQ6 * a[1] + RSQ3 * a[5];
+    T[5] = T[7] = RSQ2 * a[2];
+    T[2] = T[6] = RSQ2 * a[3];
+    T[1] = T[3] = RSQ2 * a[4];
+}
+// B-basis of the symmetric part of a 3x3 (chg_basis opt 2 of sym(T))
+__device__ __forceinline__ void sym_to_b6(const double T[9], double a[6])
+{
+    const double s00 = T[0], s11 = T[4], s22 = T[8];
+    const double s12 = (T[5] + T[7]) / 2., s02 = (T[2] + T[6]) / 2., s01 = (T[1] + T[3]) / 2.;
+    a[0] = s00 * (-RSQ2) + s11 * RSQ2;
+    a[1] = s00 * (-1. * RSQ6) + s11 * (-1. * RSQ6) + s22 * (2. * RSQ6);
+    a[2] = s12 * RSQ2 + s12 * RSQ2;
+    a[3] = s02 * RSQ2 + s02 * RSQ2;
+    a[4] = s01 * RSQ2 + s01 * RSQ2;
+    a[5] = s00 * RSQ3 + s11 * RSQ3 + s22 * RSQ3;
+}
+
+__device__ double von_mises_strain(const double t[9])
+{
+    // VonMises, evolution.c:243-262
+    const double tr = t[0] + t[4] + t[8];
+    double vm = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double d = (t[i * 3 + j] + t[j * 3 + i]) / 2. - (double)(i == j) * tr / 3.0;
+            vm += d * d;
+        }
+    return sqrt(2. / 3. * vm);
+}
+
+// get_smacro's von Mises of a stress average (evolution.c:218-238)
+__device__ double svm_of(const double S[9])
+{
+    double a[6];
+    sym_to_b6(S, a);
+    double dev[9];
+    const double a5[6] = {a[0], a[1], a[2], a[3], a[4], 0.0};
+    b6_to_t(a5, dev);
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v += dev[i] * dev[i];
+    return sqrt(3. / 2. * v);
+}
+
+// ------------------------------------------------------------------
+// Single-block finaliser of one iteration (evolution.c:405-428, 170-241)
+// ------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fin_iter(RveDev r)
+{
+    MacroState *ms = r.ms;
+    if (!iter_active(ms)) return;
+    __shared__ double tot[14];
+    __shared__ double red[NT / 64][14];
+    const int nb = r.nblk_vox;
+    double a[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) a[i] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT)
+#pragma unroll
+        for (int i = 0; i < 14; ++i) a[i] += r.part[(long)b * 16 + i];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        double x = a[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        a[i] = x;
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 14; ++i) red[threadIdx.x >> 6][i] = a[i];
+    __syncthreads();
+    if (threadIdx.x < 14) {
+        double acc = 0.0;
+        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
+        tot[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    // SigAvg, SigAvg1 (evolution.c:182-197)
+    double s6[6], s16[6], SA[9], SA1[9];
+    for (int i = 0; i < 6; ++i) { s6[i] = tot[2 + i]; s16[i] = tot[8 + i]; }
+    b6_to_t(s6, SA);
+    b6_to_t(s16, SA1);
+    for (int i = 0; i < 9; ++i) {
+        SA[i] *= ms->wgt;
+        SA1[i] *= ms->wgt_over_ph1;
+        ms->SigAvg[i] = SA[i];
+        ms->SigAvg1[i] = SA1[i];
+    }
+    // mixed-BC correction, upper triangle only (evolution.c:199-214)
+    const int IJV0[6] = {0, 1, 2, 1, 0, 0}, IJV1[6] = {0, 1, 2, 2, 2, 1};
+    double dD[9];
+    for (int i = 0; i < 9; ++i) dD[i] = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        const int ii = IJV0[i], jj = IJV1[i];
+        double acc = 0.0;
+        if (ms->velgrad_flag[i] == 0) {
+            for (int k = 0; k < 6; ++k) {
+                const int kk = IJV0[k], ll = IJV1[k];
+                acc += ms->S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * ms->stress_flag[k] *
+                       (ms->Scauchy[kk * 3 + ll] - SA[kk * 3 + ll]);
+            }
+        }
+        dD[ii * 3 + jj] = acc;
+    }
+    for (int i = 0; i < 9; ++i) {
+        ms->dD[i] = dD[i];
+        ms->acum[i] += dD[i];
+    }
+    ms->s_vm = svm_of(SA);
+    ms->s_vm1 = svm_of(SA1);
+    double Err_e = tot[1] * ms->wgt, Err_s = tot[0] * ms->wgt;
+    if (fabs(ms->e_vm) > 1E-5) Err_e /= ms->e_vm;
+    Err_s /= ms->s_vm;
+    ms->Err_e = Err_e;
+    ms->Err_s = Err_s;
+    const int it = ms->iter;
+    if (it < VOCE_MAXIT_LOG) {
+        ms->log_err_e[it] = Err_e;
+        ms->log_err_s[it] = Err_s;
+        ms->log_svm[it] = ms->s_vm;
+    }
+    ms->iter = it + 1;
+    ms->iter_total += 1;
+    const int more = (it + 1 < ms->iter_max) && (fmax(fabs(Err_s), fabs(Err_e)) > fabs(ms->err_tol));
+    if (more) {
+        for (int i = 0; i < 9; ++i) ms->applied[i] += dD[i];
+        double sh[6];
+        sym_to_b6(dD, sh);
+        for (int i = 0; i < 6; ++i) ms->shift6[i] = sh[i];
+    }
+    __threadfence();
+    ms->active = more;
+}
+
+// increment start (evolution.c:306-340)
+__global__ void k_begin_increment(RveDev r)
+{
+    MacroState *ms = r.ms;
+    ms->istep += 1;
+    ms->iter = 0;
+    for (int i = 0; i < 9; ++i) {
+        ms->dD[i] = 0.0;
+        ms->acum[i] = 0.0;
+        ms->applied[i] = 0.0;
+    }
+    if (ms->creep) {
+        double a[6];
+        sym_to_b6(ms->DisGradAvg_actual, a);
+        for (int i = 0; i < 6; ++i) {
+            ms->creep_sym6[i] = a[i];
+            ms->shift6[i] = 0.0;
+        }
+        const double *A = ms->DisGradAvg_actual;
+        ms->creep_skew[0] = (A[7] - A[5]) / 2.;
+        ms->creep_skew[1] = (A[2] - A[6]) / 2.;
+        ms->creep_skew[2] = (A[3] - A[1]) / 2.;
+    } else {
+        double U[9];
+        for (int i = 0; i < 9; ++i) U[i] = ms->Udot[i] * ms->dt;
+        double a[6];
+        sym_to_b6(U, a);
+        for (int i = 0; i < 6; ++i) ms->shift6[i] = a[i];
+    }
+    // evolution.c:338-342: Err_e = Err_s = 2 Err before the first test
+    ms->active = (ms->iter_max > 0) && (fabs(2. * ms->err_tol) > fabs(ms->err_tol));
+    ms->error = 0;
+}
+
+// Orientation, constitutive.c:117-169
+__device__ void orientation(double tran[9], const double rot_in[9])
+{
+    double rot[9];
+    for (int i = 0; i < 9; ++i) rot[i] = rot_in[i];
+    double w[3] = {rot[7], rot[2], rot[3]};
+    double w_norm = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double w_tan = tan(w_norm / 2.0);
+    if (fabs(w_norm) < 1E-6) w_norm = 1.0;
+    for (int i = 0; i < 3; ++i) w[i] *= w_tan / w_norm;
+    double om[9] = {0., -1. * w[2], w[1], w[2], 0., -1. * w[0], -1. * w[1], w[0], 0.};
+    double om2[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            om2[i * 3 + j] = om[i * 3 + 0] * om[0 * 3 + j] + om[i * 3 + 1] * om[1 * 3 + j] + om[i * 3 + 2] * om[2 * 3 + j];
+    w_tan = w_tan * w_tan;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) rot[i * 3 + j] = (double)(i == j) + 2. * (om[i * 3 + j] + om2[i * 3 + j]) / (1. + w_tan);
+    double tn[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            tn[i * 3 + j] = rot[i * 3 + 0] * tran[0 * 3 + j] + rot[i * 3 + 1] * tran[1 * 3 + j] + rot[i * 3 + 2] * tran[2 * 3 + j];
+    for (int i = 0; i < 9; ++i) tran[i] = tn[i];
+}
+
+// ------------------------------------------------------------------
+// Increment close-out per voxel (evolution.c:433-530).  Partials per
+// block: [0..4] sum Eps (B basis), [5..9] sum Edot.
+// ------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
+{
+    const MacroState *ms = r.ms;
+    const long V = r.V;
+    const long v = (long)blockIdx.x * NT + threadIdx.x;
+    double acc[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc[i] = 0.0;
+    const int istep = ms->istep;
+    const double dt = ms->dt;
+    const int diag = ms->diag;
+    if (v < V) {
+        const uint32_t ori = r.ORI[v];
+        const int ph = r.PH[v];
+        // spin of the increment: skew(DisGrad_new - DisGrad_old), components (2,1),(0,2),(1,0)
+        double dsk[3] = {0.0, 0.0, 0.0};
+        if (rot_on) {
+            const double *A = ms->applied;
+            const double sa[3] = {(A[7] - A[5]) / 2., (A[2] - A[6]) / 2., (A[3] - A[1]) / 2.};
+            double base[3];
+            if (ms->creep) {
+                for (int i = 0; i < 3; ++i) base[i] = ms->creep_skew[i] - r.WG[i * V + v];
+            } else {
+                const double *U = ms->Udot;
+                base[0] = (U[7] - U[5]) / 2. * dt;
+                base[1] = (U[2] - U[6]) / 2. * dt;
+                base[2] = (U[3] - U[1]) / 2. * dt;
+            }
+            for (int i = 0; i < 3; ++i) {
+                dsk[i] = base[i] + sa[i] + r.WF[i * V + v];
+                r.WG[i * V + v] += dsk[i];
+                r.WF[i * V + v] = dsk[i];
+            }
+        }
+        if (ori == VOCE_GAS) {
+            for (int i = 0; i < 5; ++i) r.ED5[i * V + v] = 0.0;
+            for (int i = 0; i < 5; ++i) acc[i] = r.EP5[i * V + v];
+        } else {
+            const PhaseConst *pc = r.phase + (ph - 1);
+            const int nsys = pc->nsys;
+            double Rm[9];
+            for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+            double P[12][5];
+            schmid_sample(Rm, pc, nsys, P);
+            double sp[6];
+            for (int i = 0; i < 6; ++i) sp[i] = r.P6[i * V + v];
+            double *crf = r.CRSS;
+            double *crb = pc->two_crss ? r.CRSSB : r.CRSS;
+            FlowOut fo;
+            double gd[12], tx[12];
+            flow_rule(sp, P, crf, crb, V, v, pc, nsys, fo, gd, tx);
+            double ep[5];
+            for (int i = 0; i < 5; ++i) ep[i] = r.EP5[i * V + v];
+            if (diag) {
+                // EVM from eps_tot at the pre-update stress, SVM from sigma (evolution.c:134-155)
+                const double *Sl = r.SLOC + (long)ori * 36;
+                double tt[6];
+                for (int i = 0; i < 6; ++i) {
+                    double t = (i < 5 ? ep[i] : 0.0) + (i < 5 ? fo.edot[i] : 0.0) * dt;
+                    for (int j = 0; j < 6; ++j) t += Sl[i * 6 + j] * sp[j];
+                    tt[i] = t;
+                }
+                double T[9];
+                b6_to_t(tt, T);
+                r.EVM[v] = von_mises_strain(T);
+                double s6[6];
+                for (int i = 0; i < 6; ++i) s6[i] = r.S6[i * V + v];
+                double ST[9];
+                b6_to_t(s6, ST);
+                const double tr = ST[0] + ST[4] + ST[8];
+                double sv = 0.0;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        const double d = (ST[i * 3 + j] + ST[j * 3 + i]) / 2. - (double)(i == j) * tr / 3.0;
+                        sv += d * d;
+                    }
+                r.SVM[v] = sqrt(3. / 2. * sv);
+                for (int s = 0; s < 12; ++s) {
+                    r.GAMDOT[s * V + v] = s < nsys ? gd[s] : 0.0;
+                    r.TAUX[s * V + v] = s < nsys ? tx[s] : 0.0;
+                }
+            }
+            for (int i = 0; i < 5; ++i) {
+                ep[i] += fo.edot[i] * dt;
+                r.EP5[i * V + v] = ep[i];
+                r.ED5[i * V + v] = fo.edot[i];
+                acc[i] = ep[i];
+                acc[5 + i] = fo.edot[i];
+            }
+            // update_orient (constitutive.c:171-248), from step 2
+            if (ms->update_flag == 1 && istep > 1) {
+                double Lp[9];
+                for (int i = 0; i < 9; ++i) Lp[i] = 0.0;
+                for (int s = 0; s < nsys; ++s) {
+                    double bsa[3], nsa[3];
+                    for (int i = 0; i < 3; ++i) {
+                        bsa[i] = 0.0;
+                        nsa[i] = 0.0;
+                        for (int j = 0; j < 3; ++j) {
+                            bsa[i] += Rm[i * 3 + j] * pc->bb[s][j];
+                            nsa[i] += Rm[i * 3 + j] * pc->bn[s][j];
+                        }
+                    }
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) Lp[i * 3 + j] += bsa[i] * nsa[j] * gd[s];
+                }
+                double RotLoc[9] = {0.0, -dsk[2] / dt, dsk[1] / dt, dsk[2] / dt, 0.0, -dsk[0] / dt,
+                                    -dsk[1] / dt, dsk[0] / dt, 0.0};
+                double Rot[9];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        const double rs = (Lp[i * 3 + j] - Lp[j * 3 + i]) / 2.;
+                        Rot[i * 3 + j] = (ms->Udot_a[i * 3 + j] + RotLoc[i * 3 + j] - rs) * dt;
+                    }
+                double Rn[9];
+                for (int i = 0; i < 9; ++i) Rn[i] = Rm[i];
+                orientation(Rn, Rot);
+                for (int i = 0; i < 9; ++i) r.R[i * V + v] = Rn[i];
+            }
+            // harden (constitutive.c:250-310), from step 3, and the
+            // rho/work diagnostics that follow it (evolution.c:509-529)
+            if (ms->hard_flag == 1 && istep > 2) {
+                const double gamtot = r.GAC[v];
+                double deltgam = 0.0;
+                for (int s = 0; s < nsys; ++s) {
+                    r.GTOT[s * V + v] = r.GTOT[s * V + v] + gd[s] * dt;
+                }
+                for (int s = 0; s < nsys; ++s) deltgam += fabs(gd[s]) * dt;
+                const double tau0 = pc->tau0f, tau1 = pc->tau1, thet0 = pc->thet0, thet1 = pc->thet1;
+                const double TINY = 1.E-4 * tau0;
+                double voce = 0.0;
+                if (fabs(thet0) > TINY) {
+                    voce = thet1 * deltgam;
+                    if (fabs(tau1) > TINY) {
+                        const double fact = fabs(thet0 / tau1);
+                        const double exp_ini = exp(-1.0 * gamtot * fact);
+                        const double exp_del = exp(-1.0 * deltgam * fact);
+                        voce += -1. * (fact * tau1 - thet1) / fact * exp_ini * (exp_del - 1.0) -
+                                thet1 / fact * exp_ini * (exp_del * ((gamtot + deltgam) * fact + 1.0) - (gamtot * fact + 1.0));
+                    }
+                }
+                for (int s = 0; s < nsys; ++s) {
+                    double dtau = 0.;
+                    for (int j = 0; j < nsys; ++j) dtau += pc->hard[s][j] * fabs(gd[j]) * dt;
+                    crf[s * V + v] += dtau * voce / deltgam;
+                    if (pc->two_crss) crb[s * V + v] += dtau * voce / deltgam;
+                }
+                r.GAC[v] = gamtot + deltgam;
+                if (diag) {
+                    double s6[6];
+                    for (int i = 0; i < 6; ++i) s6[i] = r.S6[i * V + v];
+                    double rr = 0.0, rc = 0.0, wp = 0.0;
+                    const double K = (0.35 * 47.565 * 0.256);
+                    for (int s = 0; s < nsys; ++s) {
+                        const double rss = P[s][0] * s6[0] + P[s][1] * s6[1] + P[s][2] * s6[2] + P[s][3] * s6[3] +
+                                           P[s][4] * s6[4];
+                        rr += (rss / K) * (rss / K);
+                        rc += (crf[s * V + v] / K) * (crf[s * V + v] / K);
+                        wp += fabs((r.GTOT[s * V + v]) * rss);
+                    }
+                    r.RHOR[v] = rr;
+                    r.RHOC[v] = rc;
+                    r.WORKP[v] = wp;
+                }
+            }
+        }
+    }
+    block_reduce_store<NT, 10>(acc, r.part + (long)blockIdx.x * 16);
+}
+
+// increment finaliser (evolution.c:438-498)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fin_incr(RveDev r)
+{
+    MacroState *ms = r.ms;
+    __shared__ double tot[10];
+    __shared__ double red[NT / 64][10];
+    double a[10];
+    for (int i = 0; i < 10; ++i) a[i] = 0.0;
+    for (int b = threadIdx.x; b < r.nblk_vox; b += NT)
+        for (int i = 0; i < 10; ++i) a[i] += r.part[(long)b * 16 + i];
+    for (int i = 0; i < 10; ++i) {
+        double x = a[i];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        a[i] = x;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int i = 0; i < 10; ++i) red[threadIdx.x >> 6][i] = a[i];
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        double acc = 0.0;
+        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
+        tot[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double dt = ms->dt;
+    for (int i = 0; i < 9; ++i) {
+        ms->DisGradAvg_actual[i] = ms->DisGradAvg[i] + ms->acum[i];
+        ms->VelGradAvg[i] = (ms->DisGradAvg_actual[i] - ms->DisGradAvg_t[i]) / dt;
+    }
+    ms->e_vm = von_mises_strain(ms->DisGradAvg_actual);
+    ms->d_vm = von_mises_strain(ms->VelGradAvg);
+    for (int i = 0; i < 9; ++i) {
+        ms->DisGradAvg_t[i] = ms->DisGradAvg_actual[i];
+        ms->DisGradAvg[i] = ms->DisGradAvg_t[i] + ms->Udot[i] * dt;
+    }
+    double e6[6] = {tot[0], tot[1], tot[2], tot[3], tot[4], 0.0};
+    double d6[6] = {tot[5], tot[6], tot[7], tot[8], tot[9], 0.0};
+    double ET[9], DT[9];
+    b6_to_t(e6, ET);
+    b6_to_t(d6, DT);
+    double evmp = 0.0, dvmp = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        ms->EpsAvg[i] = ET[i] * ms->wgt;
+        ms->EdotAvg[i] = DT[i] * ms->wgt;
+        evmp += ms->EpsAvg[i] * ms->EpsAvg[i];
+        dvmp += ms->EdotAvg[i] * ms->EdotAvg[i];
+    }
+    ms->evmp = sqrt(2. / 3. * evmp);
+    ms->dvmp = sqrt(2. / 3. * dvmp);
+}
+
+// initial fields: S6 from the per-orientation table (InitFields, init.c:486-534)
+__global__ void k_init_sig(RveDev r, const double *sig6_by_orient)
+{
+    const long V = r.V;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t ori = r.ORI[v];
+    for (int i = 0; i < 6; ++i) r.S6[i * V + v] = (ori == VOCE_GAS) ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
+}
+
+constexpr int NT_VOX = 256;
+constexpr int NT_FIN = 256;
+constexpr int NT_PL_S = 256, MAXEL_PL_S = 2112;   // planes up to 64 x 64
+constexpr int NT_PL_L = 512, MAXEL_PL_L = 8320;   // planes up to 128 x 128
+constexpr int NT_X = 256, MAXEL_X = 3072;
+
+template <typename T>
+int xgreen_tk(const RveDev *r)
+{
+    int tk = MAXEL_X / (6 * r->nx);
+    if (tk > 16) tk = 16;
+    if (tk < 1) tk = 1;
+    return tk;
+}
+
+template <typename T>
+hipError_t launch_iter_t(const RveDev *r, hipStream_t st, hipEvent_t *ev)
+{
+    using C = typename cplx_of<T>::t;
+    int evi = 0;
+#define MARK()                                                   \
+    do {                                                         \
+        if (ev) (void)hipEventRecord(ev[evi++], st);             \
+    } while (0)
+    MARK();
+    const size_t pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * sizeof(C);
+    const bool large = (r->ny * r->nzh > MAXEL_PL_S);
+    if (large) {
+        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 6), dim3(NT_PL_L), pl_lds, st, *r);
+    } else {
+        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 6), dim3(NT_PL_S), pl_lds, st, *r);
+    }
+    MARK();
+    const int tk = xgreen_tk<T>(r);
+    const int nblk = (r->ny * r->nzh + tk - 1) / tk;
+    const size_t x_lds = (size_t)(6 * tk * r->nx + r->nx) * sizeof(C);
+    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(nblk), dim3(NT_X), x_lds, st, *r, 0, tk);
+    MARK();
+    if (large) {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 6), dim3(NT_PL_L), pl_lds, st, *r, 0);
+    } else {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 6), dim3(NT_PL_S), pl_lds, st, *r, 0);
+    }
+    MARK();
+    hipLaunchKernelGGL((k_nr<NT_VOX>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+    MARK();
+    hipLaunchKernelGGL((k_fin_iter<NT_FIN>), dim3(1), dim3(NT_FIN), 0, st, *r);
+    MARK();
+#undef MARK
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_skew_t(const RveDev *r, hipStream_t st)
+{
+    using C = typename cplx_of<T>::t;
+    const int tk = xgreen_tk<T>(r);
+    const int nblk = (r->ny * r->nzh + tk - 1) / tk;
+    const size_t x_lds = (size_t)(6 * tk * r->nx + r->nx) * sizeof(C);
+    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(nblk), dim3(NT_X), x_lds, st, *r, 1, tk);
+    const size_t pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * sizeof(C);
+    if (r->ny * r->nzh > MAXEL_PL_S) {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 3), dim3(NT_PL_L), pl_lds, st, *r, 1);
+    } else {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 3), dim3(NT_PL_S), pl_lds, st, *r, 1);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+void set_lds_limits()
+{
+    const int big = 160 * 1024;
+    hipFuncSetAttribute((const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipFuncSetAttribute((const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipFuncSetAttribute((const void *)k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipFuncSetAttribute((const void *)k_plane_inv<T, NT_PL_S, MAXEL_PL_S>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipFuncSetAttribute((const void *)k_xgreen<T, NT_X, MAXEL_X>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+}
+
+}  // namespace
+
+extern "C" int voce_kernels_prepare(void)
+{
+    set_lds_limits<double>();
+    set_lds_limits<float>();
+    return 0;
+}
+
+// 2^a 3^b, 2..128, and even (the plane kernels pack row pairs)
+extern "C" int voce_grid_supported(int n)
+{
+    if (n < 2 || n > 128) return 0;
+    int m = n;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    return m == 1;
+}
+
+extern "C" int voce_launch_begin_increment(const RveDev *r, void *stream)
+{
+    hipLaunchKernelGGL(k_begin_increment, dim3(1), dim3(1), 0, (hipStream_t)stream, *r);
+    return (int)hipGetLastError();
+}
+
+extern "C" int voce_launch_iteration(const RveDev *r, void *stream, void *events)
+{
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t *ev = (hipEvent_t *)events;
+    return (int)(r->fft_f32 ? launch_iter_t<float>(r, st, ev) : launch_iter_t<double>(r, st, ev));
+}
+
+extern "C" int voce_launch_closeout(const RveDev *r, void *stream, int need_skew)
+{
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipSuccess;
+    if (need_skew) e = r->fft_f32 ? launch_skew_t<float>(r, st) : launch_skew_t<double>(r, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((k_closeout<NT_VOX>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
+    hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(1), dim3(NT_FIN), 0, st, *r);
+    return (int)hipGetLastError();
+}
+
+extern "C" int voce_launch_init_fields(const RveDev *r, const double *sig6_by_orient, void *stream)
+{
+    const long nb = (r->V + 255) / 256;
+    hipLaunchKernelGGL(k_init_sig, dim3(nb), dim3(256), 0, (hipStream_t)stream, *r, sig6_by_orient);
+    return (int)hipGetLastError();
+}

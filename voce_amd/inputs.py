@@ -1,0 +1,232 @@
+"""Input files in the reference's own formats.
+
+Writes the three inputs the reference reads -- the namelist (io.c:25-122),
+the positional ``.sx`` material file (init.c:51-187) and the 8-column voxel
+mesh (init.c:393-423) -- for seeded synthetic polycrystals:
+
+* periodic Voronoi grains: seeds uniform in [0, N)^3, voxel centres at
+  (i+0.5, j+0.5, k+0.5), minimum-image distance, ties to the lowest seed id;
+* one Bunge Euler triple per grain, uniform on SO(3):
+  phi1 = 360 u, Phi = acos(2u - 1) * 180/pi, phi2 = 360 u, written ``%.6f``;
+* all random numbers from SplitMix64 (seeds first, then Euler triples).
+
+The reference ships no mesh generator (its testf.c:97-130 only writes
+lamellae), so this module is what every synthetic case is built from.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014), 53-bit uniform doubles."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+
+def voronoi_grains(dims, n_grains: int, seed: int):
+    """Return (grain_id[V] 1-based int32, euler_deg[n_grains,3]) for a
+    periodic Voronoi tessellation of an nx*ny*nz grid (x slowest)."""
+    nx, ny, nz = dims
+    rng = SplitMix64(seed)
+    seeds = np.array([[rng.uniform() * nx, rng.uniform() * ny, rng.uniform() * nz]
+                      for _ in range(n_grains)], dtype=np.float64)
+    eul = []
+    for _ in range(n_grains):
+        p1 = 360.0 * rng.uniform()
+        ph = math.acos(2.0 * rng.uniform() - 1.0) * 180.0 / math.pi
+        p2 = 360.0 * rng.uniform()
+        eul.append((p1, ph, p2))
+    eul = np.array(eul, dtype=np.float64)
+    # round to the printed precision so that every reader sees the same angles
+    eul = np.array([[float("%.6f" % a) for a in row] for row in eul])
+    L = np.array([nx, ny, nz], dtype=np.float64)
+    gid = np.empty(nx * ny * nz, dtype=np.int32)
+    yy, zz = np.meshgrid(np.arange(ny) + 0.5, np.arange(nz) + 0.5, indexing="ij")
+    yz = np.stack([yy.ravel(), zz.ravel()], axis=1)
+    for ix in range(nx):
+        pts = np.empty((ny * nz, 3))
+        pts[:, 0] = ix + 0.5
+        pts[:, 1:] = yz
+        best = np.full(ny * nz, np.inf)
+        arg = np.zeros(ny * nz, dtype=np.int32)
+        for g in range(n_grains):
+            d = pts - seeds[g]
+            d -= L * np.round(d / L)
+            r2 = np.einsum("ij,ij->i", d, d)
+            better = r2 < best  # strict: ties stay with the lower id
+            best[better] = r2[better]
+            arg[better] = g
+        gid[ix * ny * nz:(ix + 1) * ny * nz] = arg + 1
+    return gid, eul
+
+
+def write_mesh(path, dims, grain, euler_by_grain, phase=None):
+    """8-column mesh: phi1 Phi phi2 (deg) i j k (1-based) grain phase,
+    i slowest (init.c:405-423)."""
+    nx, ny, nz = dims
+    V = nx * ny * nz
+    if phase is None:
+        phase = np.ones(V, dtype=np.int32)
+    idx = np.arange(V)
+    ii = idx // (ny * nz) + 1
+    jj = (idx // nz) % ny + 1
+    kk = idx % nz + 1
+    e = euler_by_grain[grain - 1]
+    with open(path, "w") as f:
+        for v in range(V):
+            f.write("%.6f %.6f %.6f %d %d %d %d %d\n" % (e[v, 0], e[v, 1], e[v, 2], ii[v], jj[v], kk[v],
+                                                        grain[v], phase[v]))
+
+
+FCC_SLIP = [  # Cu_p.sx:24-35, (n) (b)
+    ((1, 1, -1), (0, 1, 1)), ((1, 1, -1), (1, 0, 1)), ((1, 1, -1), (1, -1, 0)),
+    ((1, -1, -1), (0, 1, -1)), ((1, -1, -1), (1, 0, 1)), ((1, -1, -1), (1, 1, 0)),
+    ((1, -1, 1), (0, 1, 1)), ((1, -1, 1), (1, 0, -1)), ((1, -1, 1), (1, 1, 0)),
+    ((1, 1, 1), (0, 1, -1)), ((1, 1, 1), (1, 0, -1)), ((1, 1, 1), (1, -1, 0)),
+]
+
+
+@dataclass
+class Material:
+    nsrs: float = 20.0
+    gam0: float = 1.0
+    tau0f: float = 16.0
+    tau0b: float = 16.0
+    tau1: float = 2.0
+    theta0: float = 250.0
+    theta1: float = 50.0
+    slip: list = field(default_factory=lambda: list(FCC_SLIP))
+
+
+def write_material(path, m: Material):
+    """One-mode file in the positional layout PlasticInit reads (init.c:65-177)."""
+    lines = [
+        "# Crystal system (XtalSys):", "CUBIC", "",
+        "# Crystal axex (XtalAxis):", "1.0 1.0 1.0", "",
+        "# Total number of modes listed in the file (N_modes_max):", "1", "",
+        "# Number of modes to be used in the calculation (N_modes):", "1", "",
+        "# Labels of the modes to be used (iMode):", "1", "",
+        "# Mode# 1 -- <110>(111) SLIP",
+        "\tnsmx\tnrsx\tgamd0x\ttwshx\tisectwx",
+        "\t%d\t\t%r\t\t%r\t\t0.0\t\t0" % (len(m.slip), m.nsrs, m.gam0),
+        "\ttau0xf\ttau0xb\ttau1x\tthet0\tthet1",
+        "\t%r\t\t%r\t\t%r\t\t%r\t\t%r" % (m.tau0f, m.tau0b, m.tau1, m.theta0, m.theta1),
+        "\thselfx\thlatex2", "\t1.0\t\t1.0", "\tSlip (n-b)",
+    ]
+    for n, b in m.slip:
+        lines.append("\t%d %d %d\t\t%d %d %d" % (n + b))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+@dataclass
+class Case:
+    """One RVE run: grid, grains, load and solver controls."""
+    dims: tuple = (16, 16, 16)
+    n_grains: int = 8
+    seed: int = 1
+    time_step: float = 1.0
+    n_steps: int = 50
+    err: float = 1e-5
+    iter_max: int = 100
+    velgrad_flag: tuple = (1, 0, 0, 1, 1, 1)
+    velgrad: tuple = (1.0e-3, -0.5e-3, -0.5e-3, 0.0, 0.0, 0.0)
+    stress_flag: tuple = (0, 1, 1, 0, 0, 0)
+    stress: tuple = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    elast: tuple = (168400.0, 121400.0, 75400.0, 0.35)
+    material: Material = field(default_factory=Material)
+    latent: tuple = (1.0, 1.0, 1.0, 1.0, 1.0, 1.0)  # self coplane cross glissile hirth lomer
+    update_flag: int = 1
+    hard_flag: int = 1
+    print_control: tuple = (0, 1)
+    tex_flag: int = 0
+
+    @property
+    def nvox(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+
+def _fmt(v):
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def write_namelist(path, c: Case, mesh_name="mesh.txt", sx_name="material.sx"):
+    """Namelist with every key of io.c:26-51; lower-case ``hirth`` and a
+    trailing newline so that no key is dropped."""
+    kv = [
+        ("N_phases", "2"),
+        ("CellDim", " ".join(map(str, c.dims))),
+        ("initial_ms", mesh_name),
+        ("Type_phases", "0 1"),
+        ("ElastConst_PhaseI", " ".join(_fmt(float(x)) for x in c.elast)),
+        ("Slip_PhaseI", sx_name),
+        ("ElastConst_PhaseII", "0.0 0.0 0.0 0.0"),
+        ("Slip_PhaseII", "XXXXX"),
+        ("VelGrad_BC_Flag", " ".join(map(str, c.velgrad_flag))),
+        ("VelGrad_BC", " ".join(_fmt(float(x)) for x in c.velgrad)),
+        ("Stress_BC_Flag", " ".join(map(str, c.stress_flag))),
+        ("Stress_BC", " ".join(_fmt(float(x)) for x in c.stress)),
+        ("TimeStep", _fmt(float(c.time_step))),
+        ("N_steps", str(c.n_steps)),
+        ("Err", _fmt(float(c.err))),
+        ("IterMax", str(c.iter_max)),
+        ("PrintControl", " ".join(map(str, c.print_control))),
+        ("Update_Flag", str(c.update_flag)),
+        ("Hard_Flag", str(c.hard_flag)),
+        ("Tex_Flag", str(c.tex_flag)),
+    ]
+    names = ("self", "coplane", "cross", "glissile", "hirth", "lomer")
+    for n, v in zip(names, c.latent):
+        kv.append((n, "%s 0.0" % _fmt(float(v))))
+    with open(path, "w") as f:
+        f.write("# synthetic case: %s grid, %d grains, seed %d\n" % ("x".join(map(str, c.dims)),
+                                                                     c.n_grains, c.seed))
+        for k, v in kv:
+            f.write("%s\t%s\n" % (k, v))
+
+
+def write_case(directory, c: Case, grain=None, euler=None):
+    """Write input, material.sx and mesh.txt for case ``c`` into ``directory``;
+    returns the namelist path."""
+    os.makedirs(directory, exist_ok=True)
+    if grain is None:
+        grain, euler = voronoi_grains(c.dims, c.n_grains, c.seed)
+    write_material(os.path.join(directory, "material.sx"), c.material)
+    write_mesh(os.path.join(directory, "mesh.txt"), c.dims, grain, euler)
+    p = os.path.join(directory, "input")
+    write_namelist(p, c)
+    return p
+
+
+def config_case(n: int) -> Case:
+    """The workloads the metric is quoted on (16^3 CPU case; 64^3 single RVE;
+    128^3 plane-strain compression)."""
+    if n == 1:
+        return Case(dims=(16, 16, 16), n_grains=8, seed=1, time_step=1.0, n_steps=50)
+    if n == 2:
+        return Case(dims=(64, 64, 64), n_grains=125, seed=2, time_step=0.5, n_steps=100)
+    if n == 3:
+        return Case(dims=(128, 128, 128), n_grains=1000, seed=3, time_step=1.0, n_steps=100,
+                    velgrad_flag=(1, 1, 1, 1, 1, 1), velgrad=(1.0e-3, 0.0, -1.0e-3, 0.0, 0.0, 0.0),
+                    stress_flag=(0, 0, 0, 0, 0, 0))
+    raise ValueError(n)
