@@ -374,6 +374,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     const int nx = su->dims[0], ny = su->dims[1], nz = su->dims[2];
     if (nx <= 0 || ny <= 0 || nz <= 0 || su->n_phases < 1 || su->n_phases > 2) return VOCE_ERR_ARG;
     if (!voce_grid_supported(nx) || !voce_grid_supported(ny) || !voce_grid_supported(nz)) return VOCE_ERR_UNSUPPORTED_GRID;
+    if (ny % 2 || nz % 2) return VOCE_ERR_UNSUPPORTED_GRID;   // the plane kernels pack row pairs
     if (su->iter_max > VOCE_MAXIT_LOG) return VOCE_ERR_ARG;
     const long V = (long)nx * ny * nz;
     const double WGT = 1. / V;

@@ -27,7 +27,8 @@
 // FFT type T: double2 in FP64 mode, float2 in MIXED mode):
 //   SPEC  6 x Nx x Ny x NZH   index ((c*Nx + kx)*Ny + ky)*NZH + kz
 //   USUM  3 x Nx x Ny x NZH   sum over the increment's iterations of the
-//                             Green displacement u_hat (skew recovery)
+//                             skew part of the Green update, components
+//                             (2,1), (0,2), (1,0) (lattice-spin recovery)
 #pragma once
 #include <stdint.h>
 

@@ -370,11 +370,76 @@ __device__ __forceinline__ void kvec(int kx, int ky, int kz, int nx, int ny, int
     n[2] = (T)(t2 * ig);
 }
 
+// Green operator applied to one k point (init.c:595-610, evolution.c:363-377):
+// u = -omega(n) (sigma_hat n) / V with omega = (C0_ijkl n_j n_l)^-1 by
+// cofactors; d = B-basis components of sym(u (x) n); w = skew(u (x) n)
+// components (2,1), (0,2), (1,0).
+template <typename T, typename C>
+__device__ __forceinline__ void green_apply(const double *C0, const T n[3], const C a[6], T invV, C d[6], C w[3])
+{
+    T K[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            T acc = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) acc += (T)C0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
+            K[i][k] = acc;
+        }
+    const T det = K[0][0] * (K[1][1] * K[2][2] - K[1][2] * K[2][1]) - K[0][1] * (K[1][0] * K[2][2] - K[1][2] * K[2][0]) +
+                  K[0][2] * (K[1][0] * K[2][1] - K[2][0] * K[1][1]);
+    const T id = (T)1 / det;
+    T om[3][3];
+    om[0][0] = (K[1][1] * K[2][2] - K[1][2] * K[2][1]) * id;
+    om[0][1] = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) * id;
+    om[0][2] = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) * id;
+    om[1][0] = (K[1][2] * K[2][0] - K[1][0] * K[2][2]) * id;
+    om[1][1] = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) * id;
+    om[1][2] = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) * id;
+    om[2][0] = (K[1][0] * K[2][1] - K[1][1] * K[2][0]) * id;
+    om[2][1] = (K[0][1] * K[2][0] - K[0][0] * K[2][1]) * id;
+    om[2][2] = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) * id;
+    const T q2 = (T)RSQ2, q3 = (T)RSQ3, q6 = (T)RSQ6;
+    C sg[3][3];
+    sg[0][0] = C{-q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, -q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
+    sg[1][1] = C{q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
+    sg[2][2] = C{(T)2 * q6 * a[1].x + q3 * a[5].x, (T)2 * q6 * a[1].y + q3 * a[5].y};
+    sg[1][2] = sg[2][1] = C{q2 * a[2].x, q2 * a[2].y};
+    sg[0][2] = sg[2][0] = C{q2 * a[3].x, q2 * a[3].y};
+    sg[0][1] = sg[1][0] = C{q2 * a[4].x, q2 * a[4].y};
+    C tv[3], u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        tv[i] = C{(sg[i][0].x * n[0] + sg[i][1].x * n[1] + sg[i][2].x * n[2]) * invV,
+                  (sg[i][0].y * n[0] + sg[i][1].y * n[1] + sg[i][2].y * n[2]) * invV};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        u[i] = C{-(om[i][0] * tv[0].x + om[i][1] * tv[1].x + om[i][2] * tv[2].x),
+                 -(om[i][0] * tv[0].y + om[i][1] * tv[1].y + om[i][2] * tv[2].y)};
+#define UN(i, j) C{u[i].x * n[j], u[i].y * n[j]}
+    const C u00 = UN(0, 0), u11 = UN(1, 1), u22 = UN(2, 2);
+    const C u12 = UN(1, 2), u21 = UN(2, 1), u02 = UN(0, 2), u20 = UN(2, 0), u01 = UN(0, 1), u10 = UN(1, 0);
+#undef UN
+    d[0] = C{q2 * (u11.x - u00.x), q2 * (u11.y - u00.y)};
+    d[1] = C{q6 * ((T)2 * u22.x - u00.x - u11.x), q6 * ((T)2 * u22.y - u00.y - u11.y)};
+    d[2] = C{q2 * (u12.x + u21.x), q2 * (u12.y + u21.y)};
+    d[3] = C{q2 * (u02.x + u20.x), q2 * (u02.y + u20.y)};
+    d[4] = C{q2 * (u01.x + u10.x), q2 * (u01.y + u10.y)};
+    d[5] = C{q3 * (u00.x + u11.x + u22.x), q3 * (u00.y + u11.y + u22.y)};
+    w[0] = C{(T)0.5 * (u21.x - u12.x), (T)0.5 * (u21.y - u12.y)};
+    w[1] = C{(T)0.5 * (u02.x - u20.x), (T)0.5 * (u02.y - u20.y)};
+    w[2] = C{(T)0.5 * (u10.x - u01.x), (T)0.5 * (u10.y - u01.y)};
+}
+
 // ------------------------------------------------------------------
 // x-direction FFT + Green operator.  grid ceil(ny*nzh / tk)
 // mode 0 (iteration): SPEC (6 comps) -> fwd x FFT -> Delta = B(sym(u (x) n))
-//   with u = -omega(n) sigma_hat n / V -> inverse x FFT -> SPEC; USUM += u.
-// mode 1 (skew): USUM (3 comps) -> skew(u (x) n) -> inverse x FFT -> SPEC[0..2]
+//   with u = -omega(n) sigma_hat n / V -> inverse x FFT -> SPEC;
+//   USUM += skew(u (x) n).
+// mode 1 (skew): USUM (3 comps, already in k space) -> inverse x FFT -> SPEC[0..2]
 // ------------------------------------------------------------------
 template <typename T, int NT, int MAXEL>
 __global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
@@ -410,97 +475,49 @@ __global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
         const int t = e / nx, x = e - t * nx;
         const int col = col0 + t;
         const int ky = col / nzh, kz = col - ky * nzh;
-        T n[3];
-        const bool zero = (x == 0 && ky == 0 && kz == 0);
-        if (!zero) kvec<T>(x, ky, kz, nx, ny, nz, n);
-        else n[0] = n[1] = n[2] = (T)0;
         const long gidx = (long)x * ncolt + col;
         if (mode == 0) {
             C a[6];
 #pragma unroll
             for (int cc = 0; cc < 6; ++cc) a[cc] = s[(cc * tk + t) * nx + x];
-            C u[3] = {C{0, 0}, C{0, 0}, C{0, 0}};
-            if (!zero) {
-                // omega^-1_ik = C0_ijkl n_j n_l, init.c:595-602
-                T K[3][3];
+            C d[6], w[3];
+            if (x == 0 && ky == 0 && kz == 0) {
+                // Gamma(k = 0) = 0 (init.c:586-591)
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
+                for (int cc = 0; cc < 6; ++cc) d[cc] = C{0, 0};
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        T acc = 0;
+                for (int cc = 0; cc < 3; ++cc) w[cc] = C{0, 0};
+            } else {
+                T n[3];
+                kvec<T>(x, ky, kz, nx, ny, nz, n);
+                green_apply<T, C>(ms->C0, n, a, invV, d, w);
+                // The reference keeps the full spectrum and takes Re() of the
+                // inverse (evolution.c:383-391), i.e. it applies the Hermitian
+                // average of Gamma at k and at the index of -k.  With round()
+                // mapping index N/2 to -pi (init.c:661-667) the two differ on
+                // the x and y Nyquist planes; the z half-spectrum stores only
+                // one of k, -k for 0 < kz < Nz/2, so average there explicitly
+                // (kz = 0 and Nz/2 are averaged by the real-part extraction).
+                const bool nyq = (2 * x == nx) || (2 * ky == ny);
+                if (nyq && kz > 0 && 2 * kz < nz) {
+                    T m[3];
+                    kvec<T>((nx - x) % nx, (ny - ky) % ny, nz - kz, nx, ny, nz, m);
+                    C d2[6], w2[3];
+                    green_apply<T, C>(ms->C0, m, a, invV, d2, w2);
 #pragma unroll
-                        for (int j = 0; j < 3; ++j)
+                    for (int cc = 0; cc < 6; ++cc) d[cc] = C{(T)0.5 * (d[cc].x + d2[cc].x), (T)0.5 * (d[cc].y + d2[cc].y)};
 #pragma unroll
-                            for (int l = 0; l < 3; ++l) acc += (T)ms->C0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
-                        K[i][k] = acc;
-                    }
-                // M3Inverse, init.c:546-568
-                const T det = K[0][0] * (K[1][1] * K[2][2] - K[1][2] * K[2][1]) -
-                              K[0][1] * (K[1][0] * K[2][2] - K[1][2] * K[2][0]) +
-                              K[0][2] * (K[1][0] * K[2][1] - K[2][0] * K[1][1]);
-                const T id = (T)1 / det;
-                T om[3][3];
-                om[0][0] = (K[1][1] * K[2][2] - K[1][2] * K[2][1]) * id;
-                om[0][1] = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) * id;
-                om[0][2] = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) * id;
-                om[1][0] = (K[1][2] * K[2][0] - K[1][0] * K[2][2]) * id;
-                om[1][1] = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) * id;
-                om[1][2] = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) * id;
-                om[2][0] = (K[1][0] * K[2][1] - K[1][1] * K[2][0]) * id;
-                om[2][1] = (K[0][1] * K[2][0] - K[0][0] * K[2][1]) * id;
-                om[2][2] = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) * id;
-                // sigma_hat tensor from its B-basis components
-                const T q2 = (T)RSQ2, q3 = (T)RSQ3, q6 = (T)RSQ6;
-                C sg[3][3];
-                sg[0][0] = C{-q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, -q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
-                sg[1][1] = C{q2 * a[0].x - q6 * a[1].x + q3 * a[5].x, q2 * a[0].y - q6 * a[1].y + q3 * a[5].y};
-                sg[2][2] = C{(T)2 * q6 * a[1].x + q3 * a[5].x, (T)2 * q6 * a[1].y + q3 * a[5].y};
-                sg[1][2] = sg[2][1] = C{q2 * a[2].x, q2 * a[2].y};
-                sg[0][2] = sg[2][0] = C{q2 * a[3].x, q2 * a[3].y};
-                sg[0][1] = sg[1][0] = C{q2 * a[4].x, q2 * a[4].y};
-                C tv[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    tv[i] = C{(sg[i][0].x * n[0] + sg[i][1].x * n[1] + sg[i][2].x * n[2]) * invV,
-                              (sg[i][0].y * n[0] + sg[i][1].y * n[1] + sg[i][2].y * n[2]) * invV};
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    u[i] = C{-(om[i][0] * tv[0].x + om[i][1] * tv[1].x + om[i][2] * tv[2].x),
-                             -(om[i][0] * tv[0].y + om[i][1] * tv[1].y + om[i][2] * tv[2].y)};
+                    for (int cc = 0; cc < 3; ++cc) w[cc] = C{(T)0.5 * (w[cc].x + w2[cc].x), (T)0.5 * (w[cc].y + w2[cc].y)};
+                }
             }
-            // Delta grad u = u_i n_j; its symmetric part in the B basis
-            C d[6];
-#define UN(i, j) C{u[i].x * n[j], u[i].y * n[j]}
-            {
-                const T q2 = (T)RSQ2, q3 = (T)RSQ3, q6 = (T)RSQ6;
-                C u00 = UN(0, 0), u11 = UN(1, 1), u22 = UN(2, 2);
-                C u12 = UN(1, 2), u21 = UN(2, 1), u02 = UN(0, 2), u20 = UN(2, 0), u01 = UN(0, 1), u10 = UN(1, 0);
-                d[0] = C{q2 * (u11.x - u00.x), q2 * (u11.y - u00.y)};
-                d[1] = C{q6 * ((T)2 * u22.x - u00.x - u11.x), q6 * ((T)2 * u22.y - u00.y - u11.y)};
-                d[2] = C{q2 * (u12.x + u21.x), q2 * (u12.y + u21.y)};
-                d[3] = C{q2 * (u02.x + u20.x), q2 * (u02.y + u20.y)};
-                d[4] = C{q2 * (u01.x + u10.x), q2 * (u01.y + u10.y)};
-                d[5] = C{q3 * (u00.x + u11.x + u22.x), q3 * (u00.y + u11.y + u22.y)};
-            }
-#undef UN
 #pragma unroll
             for (int cc = 0; cc < 6; ++cc) s[(cc * tk + t) * nx + x] = d[cc];
+            // sum of the skew parts over the increment (lattice spin)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 C *up = usum + i * cstride + gidx;
-                *up = first ? u[i] : cadd(*up, u[i]);
+                *up = first ? w[i] : cadd(*up, w[i]);
             }
-        } else {
-            C u[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) u[i] = s[(i * tk + t) * nx + x];
-            // skew(u (x) n): components (2,1), (0,2), (1,0)
-            C w0 = C{(T)0.5 * (u[2].x * n[1] - u[1].x * n[2]), (T)0.5 * (u[2].y * n[1] - u[1].y * n[2])};
-            C w1 = C{(T)0.5 * (u[0].x * n[2] - u[2].x * n[0]), (T)0.5 * (u[0].y * n[2] - u[2].y * n[0])};
-            C w2 = C{(T)0.5 * (u[1].x * n[0] - u[0].x * n[1]), (T)0.5 * (u[1].y * n[0] - u[0].y * n[1])};
-            s[(0 * tk + t) * nx + x] = w0;
-            s[(1 * tk + t) * nx + x] = w1;
-            s[(2 * tk + t) * nx + x] = w2;
         }
     }
     __syncthreads();
@@ -1368,7 +1385,7 @@ extern "C" int voce_kernels_prepare(void)
     return 0;
 }
 
-// 2^a 3^b, 2..128, and even (the plane kernels pack row pairs)
+// 2^a 3^b, 2..128 (y and z must also be even: the plane kernels pack row pairs)
 extern "C" int voce_grid_supported(int n)
 {
     if (n < 2 || n > 128) return 0;
