@@ -756,6 +756,7 @@ struct orc_job {
     t2 *kSig_r, *kSig_i;
     cpx *fft;
     double *err_e_v, *err_s_v;
+    double *nr_iters;     /* Newton iterations of each voxel's last UpdateStress */
     int error;
 };
 
@@ -928,7 +929,7 @@ void orc_free(orc_job *J)
     free(J->SVM); free(J->EVM); free(J->work_p); free(J->rho_rss); free(J->rho_crss);
     free(J->Sig); free(J->Eps); free(J->Edot); free(J->DisGrad); free(J->VelGrad);
     free(J->fluctuation); free(J->kSig_r); free(J->kSig_i); free(J->fft);
-    free(J->err_e_v); free(J->err_s_v);
+    free(J->err_e_v); free(J->err_s_v); free(J->nr_iters);
     free(J);
 }
 
@@ -978,6 +979,7 @@ orc_job *orc_setup(const char *input_path, const char *base_dir, int *err)
         J->kSig_i = calloc(V, sizeof(t2));
         J->fft = calloc(V, sizeof(cpx));
         J->err_e_v = calloc(V, sizeof(double));
+        J->nr_iters = calloc(V, sizeof(double));
         J->err_s_v = calloc(V, sizeof(double));
     }
     /* material, init.c:696-703 */
@@ -1090,7 +1092,7 @@ static int update_stress_voxel(const double C_gr[6][6], const double C066[6][6],
                                const t2 DisGrad, const double sc[NSYSMX][5], const double tau_t[NSYSMX][2],
                                int nsys, const double *nsr, const double *gam0, double dt, t2 Edot,
                                double *err_e, double *err_s, double *SVMp, double *EVMp,
-                               double *gamdot, double *taux)
+                               double *gamdot, double *taux, double *niter)
 {
     double sg66[6][6], jacob_inv[6][6], d_edotp66[6][6];
     t2 xlambda_aux, sig_aux, eps_aux, strain_aux, edotp_aux, tot_e, SigDev, eDev;
@@ -1156,6 +1158,7 @@ static int update_stress_voxel(const double C_gr[6][6], const double C066[6][6],
         conv_e = sqrt(conv_e);
         conv_s = sqrt(conv_s);
     }
+    if (niter) *niter = iterl;
     chg_basis(sig6, sig_aux, NULL, NULL, 1);
     chg_basis(edotp6, edotp_aux, NULL, NULL, 1);
     chg_basis(tot_eps, tot_e, NULL, NULL, 1);
@@ -1195,7 +1198,7 @@ int orc_update_stress_voxel(const double C_gr[36], const double C066[36], const 
     return update_stress_voxel((const double(*)[6])C_gr, (const double(*)[6])C066, (double(*)[3])Sig_out,
                                (const double(*)[3])Eps, (const double(*)[3])DisGrad, (const double(*)[5])schm,
                                (const double(*)[2])tau_t, nsys, nsr, gam0, dt, (double(*)[3])Edot_out, err_e,
-                               err_s, svm, evm, gamdot, taux);
+                               err_s, svm, evm, gamdot, taux, NULL);
 }
 
 /* UpdateStress, evolution.c:3-168: voxel loop, then the serial error sum */
@@ -1211,7 +1214,7 @@ static int update_stress(orc_job *J, double *Err_e_local, double *Err_s_local)
             int rc = update_stress_voxel(J->C_gr[p], J->C066, J->Sig[p], J->Eps[p], J->DisGrad[p], J->Schm_gr[p],
                                          J->crss[p], J->nSYS[jph - 1], J->nSRS[jph - 1], J->gam0[jph - 1],
                                          J->nm.TimeStep, J->Edot[p], &J->err_e_v[p], &J->err_s_v[p], &J->SVM[p],
-                                         &J->EVM[p], J->gamdot[p], J->tau_x[p]);
+                                         &J->EVM[p], J->gamdot[p], J->tau_x[p], &J->nr_iters[p]);
             if (rc) fail |= 1;
         } else {
             memset(J->Sig[p], 0, sizeof(t2));
@@ -1551,6 +1554,9 @@ int orc_get_field(orc_job *J, int field, double *out)
         memcpy(out, f, sizeof(double) * V);
         return 1;
     }
+    case 19: /* Newton iterations of the last UpdateStress (diagnostic) */
+        memcpy(out, J->nr_iters, sizeof(double) * V);
+        return 1;
     case F_SCHMID:
         for (c = 0; c < 60; c++)
             for (p = 0; p < V; p++) out[c * V + p] = J->Schm_gr[p][c / 5][c % 5];

@@ -3,6 +3,8 @@
   voce_amd/libvoce.so   HIP kernels + host context + file formats (the C ABI
                         of include/voce.h)
   voce_amd/bin/voce     the command-line host driver
+  voce_amd/bin/hbm_copy a plain copy kernel of known size (traffic-counter
+                        calibration)
 
 Run ``python -m voce_amd.build``.  hipcc cross-compiles without a GPU.
 """
@@ -42,6 +44,8 @@ def build(verbose_resources=False):
     run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, k_o, c_o, io_o, "-o", lib])
     run(["gcc", "-O2", "-std=gnu99", "-Wall", os.path.join(CSRC, "voce_main.c"), "-o",
          os.path.join(HERE, "bin", "voce"), "-L" + HERE, "-lvoce", "-Wl,-rpath,$ORIGIN/.."])
+    # calibration kernel for the HBM traffic counters
+    run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "hbm_copy.hip"), "-o", os.path.join(HERE, "bin", "hbm_copy")])
     return lib
 
 

@@ -352,13 +352,15 @@ static int setup_phase(const voce_phase &in, PhaseConst &pc, double C4[3][3][3][
         for (int m = 0; m < 3; ++m) { bn[m] = in.bn[s][m] / nn; bb[m] = in.bb[s][m] / nb; }
         for (int m = 0; m < 3; ++m) dot += bn[m] * bb[m];
         if (dot > 1E-3) return VOCE_ERR_SLIP_NOT_ORTHOGONAL;   // init.c:213-215
-        double a33[3][3], a5[5], p33[3][3];
+        double a33[3][3], a5[5];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) a33[i][j] = (bn[i] * bb[j] + bn[j] * bb[i]) / 2.0;
         t_to_v(a33, a5, 5);    // Schm_xt (init.c:218-224)
+        for (int i = 0; i < 5; ++i) pc.px[s][i] = a5[i];
+        double p33[3][3];
         v_to_t(a5, p33, 5);    // as update_schmid expands it (kinematics.c:440-441)
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) pc.P3[s][i * 3 + j] = p33[i][j];
+        pc.p3[s][0] = p33[0][0]; pc.p3[s][1] = p33[1][1]; pc.p3[s][2] = p33[2][2];
+        pc.p3[s][3] = p33[1][2]; pc.p3[s][4] = p33[0][2]; pc.p3[s][5] = p33[0][1];
         for (int m = 0; m < 3; ++m) { pc.bn[s][m] = bn[m]; pc.bb[s][m] = bb[m]; }
     }
     for (int i = 0; i < VOCE_NSYS; ++i)
@@ -450,6 +452,28 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     if (lu_inv66(S066)) { free(ms0); return VOCE_ERR_SINGULAR; }
     c66_to_c4(C066, C0);
     c66_to_c4(S066, S0);
+    // per-orientation table (stride VOCE_OTAB): S_local = C_gr^-1, then
+    // A^-1 and A^-1 C066 with A = I + C066 S_local, the constant part of the
+    // Newton Jacobian of UpdateStress (evolution.c:85-93)
+    std::vector<double> otab(VOCE_OTAB * (norient > 0 ? norient : 1), 0.0);
+    for (long o = 0; o < norient; ++o) {
+        double A[6][6];
+        for (int i = 0; i < 36; ++i) otab[VOCE_OTAB * o + i] = Sloc[36 * o + i];
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double t = (double)(i == j);
+                for (int k = 0; k < 6; ++k) t += C066[i][k] * Sloc[36 * o + k * 6 + j];
+                A[i][j] = t;
+            }
+        if (lu_inv66(A)) { free(ms0); return VOCE_ERR_SINGULAR; }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                otab[VOCE_OTAB * o + 36 + i * 6 + j] = A[i][j];
+                double t = 0.0;
+                for (int k = 0; k < 6; ++k) t += A[i][k] * C066[k][j];
+                otab[VOCE_OTAB * o + 72 + i * 6 + j] = t;
+            }
+    }
 
     // boundary conditions (init.c:713-745)
     const int *vf = su->velgrad_flag, *sf = su->stress_flag;
@@ -561,6 +585,9 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     d.V = V;
     d.nspec = (long)nx * ny * d.nzh;
     d.fft_f32 = (su->precision == VOCE_MIXED);
+    // one crystalline phase, numbered 1: its constants are uniform across a wave
+    d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
+    d.two_crss = two_any;
     const size_t cb = d.fft_f32 ? 8 : 16;
     const size_t db = sizeof(double);
     ALLOC(d.E6, 6 * V * db);
@@ -578,7 +605,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     ALLOC(d.GTOT, 12 * V * db);
     ALLOC(d.ORI, V * sizeof(uint32_t));
     ALLOC(d.PH, V);
-    ALLOC(d.SLOC, 36 * (norient > 0 ? norient : 1) * db);
+    ALLOC(d.SLOC, otab.size() * db);
     ALLOC(d.phase, 2 * sizeof(PhaseConst));
     ALLOC(d.ms, sizeof(MacroState));
     ALLOC(d.spec, 6 * d.nspec * cb);
@@ -617,7 +644,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
         H2D(d.R, Rv.data(), 9 * V * db);
         H2D(d.ORI, ori.data(), V * sizeof(uint32_t));
         H2D(d.PH, phv.data(), V);
-        if (norient) H2D(d.SLOC, Sloc.data(), 36 * norient * db);
+        H2D(d.SLOC, otab.data(), otab.size() * db);
         H2D(d.phase, pcs, 2 * sizeof(PhaseConst));
         H2D(d.ms, ms0, sizeof(MacroState));
         H2D(sig6o_d, sig6o.data(), sig6o.size() * db);

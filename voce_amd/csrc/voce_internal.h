@@ -20,7 +20,8 @@
 //                      Green-operator part until the close-out completes it)
 //   GAC      V f64     accumulated shear gamacum;  GTOT 12 x V gammatotal
 //   ORI      V u32     index of the voxel's initial orientation in the
-//                      per-orientation compliance table (0xffffffff = gas)
+//                      per-orientation table (S_local, A^-1, A^-1 C066 with
+//                      A = I + C066 S_local; 0xffffffff = gas)
 //   PH       V u8      phase id (1-based)
 //
 // FFT work buffers (half spectrum along z, NZH = Nz/2 + 1, complex of the
@@ -34,6 +35,7 @@
 
 #define VOCE_MAXIT_LOG 4096
 #define VOCE_GAS 0xffffffffu
+#define VOCE_OTAB 108   // doubles per orientation-table row
 
 struct PhaseConst {
     int nsys;
@@ -41,7 +43,8 @@ struct PhaseConst {
     int two_crss;   // 1 when forward and backward CRSS are stored separately
     int int_n;      // 1 when nsrs is an integer in [1, 64]
     double nsr, gam0, tau0f, tau0b, tau1, thet0, thet1;
-    double P3[12][9];      // crystal-frame Schmid tensors (chg_basis5 opt 1 of Schm_xt)
+    double px[12][5];      // crystal-frame Schmid vectors Schm_xt (init.c:217-224)
+    double p3[12][6];      // the same as symmetric tensors: 00 11 22 12 02 01
     double bn[12][3], bb[12][3];
     double hard[12][12];
 };
@@ -69,7 +72,7 @@ struct MacroState {
     double creep_sym6[6];    // creep mode: uniform sym(DisGrad) at predictor
     double creep_skew[3];    // creep mode: uniform skew(DisGrad) at predictor
     int istep, iter, iter_total, active;
-    int error, pad1, pad2, pad3;
+    int error, ticket, pad2, pad3;   // ticket: blocks of k_nr done
     double log_err_e[VOCE_MAXIT_LOG], log_err_s[VOCE_MAXIT_LOG], log_svm[VOCE_MAXIT_LOG];
 };
 
@@ -81,7 +84,8 @@ struct RveDev {
     double *GAMDOT, *TAUX, *SVM, *EVM, *WORKP, *RHOR, *RHOC;   // diagnostics (may be null)
     uint32_t *ORI;
     uint8_t *PH;
-    double *SLOC;                // per orientation: S_local = C_gr^-1, 36 doubles
+    double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1,
+                                 // A^-1 and A^-1 C066 (36 each), A = I + C066 S_local
     PhaseConst *phase;           // [2]
     MacroState *ms;
     void *spec, *usum;           // complex T
@@ -89,6 +93,8 @@ struct RveDev {
     int nblk_vox;                // blocks of the voxel kernels
     void *tw_x, *tw_y, *tw_z;    // twiddle tables, complex T, length N, exp(-2 pi i j/N)
     int fft_f32;                 // 1: FFT in FP32
+    int uniform_phase;           // 1: one crystalline phase (its constants are uniform)
+    int two_crss;                // 1: some phase keeps a separate backward CRSS
 };
 
 #ifdef __cplusplus

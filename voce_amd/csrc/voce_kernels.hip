@@ -1,6 +1,6 @@
 // voce_kernels.hip -- CDNA4 (gfx950) kernels of one EVP-FFT increment.
 //
-// One FFT fixed-point iteration (evolution.c:342-431) is five launches:
+// One FFT fixed-point iteration (evolution.c:342-431) is four launches:
 //   k_plane_fwd  per (x-plane, stress component): real-to-half-complex FFT
 //                along z (two real rows packed per complex row) and complex
 //                FFT along y, all in LDS.                       (evolution.c:350-361)
@@ -12,12 +12,12 @@
 //                E += dDisGradAvg + IFFT/N (evolution.c:383-394).
 //   k_nr         one voxel per lane: the 6x6 Newton-Raphson of UpdateStress
 //                (evolution.c:3-168) with StrainRate_eval (constitutive.c:4-72)
-//                and the LU of LU_inv_66 (kinematics.c:282-424) in registers,
-//                plus wavefront-shuffle partial sums of the error norms and
-//                the stress averages.
-//   k_fin_iter   one block: fixed-order sum of the partials, get_smacro
-//                (evolution.c:170-241), error normalisation and the
-//                convergence test (evolution.c:342, 405-428).
+//                evaluated in the crystal frame and the LU of LU_inv_66
+//                (kinematics.c:282-424) in registers, wavefront-shuffle
+//                partial sums of the error norms and stress averages; the
+//                last block to finish sums them in fixed order and runs
+//                get_smacro (evolution.c:170-241), the error normalisation
+//                and the convergence test (evolution.c:342, 405-428).
 // Every launch reads MacroState::active first, so iterations enqueued past
 // convergence return at once and the host never blocks inside a loop.
 //
@@ -535,37 +535,54 @@ __global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
 // Constitutive pieces (double, registers)
 // ------------------------------------------------------------------
 
-// Schm_gr of update_schmid (kinematics.c:426-459): P = R P_xtal R^T,
-// projected on b1..b5.
-__device__ __forceinline__ void schmid_sample(const double R[9], const PhaseConst *pc, int nsys,
-                                              double P[12][5])
+// Rotation of the deviatoric B basis by R (crystal -> sample):
+// Q[a][b] = B_a : (R B_b R^T), a, b < 5.  A crystal-frame Schmid vector p
+// maps to the sample frame as Q p, so update_schmid (kinematics.c:426-459)
+// is Q applied to Schm_xt, and the resolved shear is p . (Q^T sigma).
+__device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5])
 {
+    // columns of R
+    const double c0[3] = {R[0], R[3], R[6]}, c1[3] = {R[1], R[4], R[7]}, c2[3] = {R[2], R[5], R[8]};
 #pragma unroll
-    for (int s = 0; s < 12; ++s) {
-        if (s < nsys) {
-            const double *a = pc->P3[s];
-            double M[9], Q[9];
+    for (int b = 0; b < 5; ++b) {
+        // M = R B_b R^T, symmetric: entries 00 11 22 12 02 01
+        double m[6];
+#define DY(u, v, i, j) ((u)[i] * (v)[j] + (v)[i] * (u)[j])
+        if (b == 0) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 6; ++q) {
+                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
+                m[q] = RSQ2 * (c1[i] * c1[j] - c0[i] * c0[j]);
+            }
+        } else if (b == 1) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    M[i * 3 + j] = R[i * 3 + 0] * a[0 * 3 + j] + R[i * 3 + 1] * a[1 * 3 + j] + R[i * 3 + 2] * a[2 * 3 + j];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    Q[i * 3 + j] = M[i * 3 + 0] * R[j * 3 + 0] + M[i * 3 + 1] * R[j * 3 + 1] + M[i * 3 + 2] * R[j * 3 + 2];
-            P[s][0] = Q[0] * (-RSQ2) + Q[4] * RSQ2;
-            P[s][1] = Q[0] * (-1. * RSQ6) + Q[4] * (-1. * RSQ6) + Q[8] * (2. * RSQ6);
-            P[s][2] = Q[5] * RSQ2 + Q[7] * RSQ2;
-            P[s][3] = Q[2] * RSQ2 + Q[6] * RSQ2;
-            P[s][4] = Q[1] * RSQ2 + Q[3] * RSQ2;
+            for (int q = 0; q < 6; ++q) {
+                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
+                m[q] = RSQ6 * (2. * c2[i] * c2[j] - c0[i] * c0[j] - c1[i] * c1[j]);
+            }
         } else {
+            const double *u = (b == 2) ? c1 : c0;
+            const double *w = (b == 4) ? c1 : c2;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) P[s][k] = 0.0;
+            for (int q = 0; q < 6; ++q) {
+                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
+                m[q] = RSQ2 * DY(u, w, i, j);
+            }
         }
+#undef DY
+        Q[0][b] = RSQ2 * (m[1] - m[0]);
+        Q[1][b] = RSQ6 * (2. * m[2] - m[0] - m[1]);
+        Q[2][b] = 2. * RSQ2 * m[3];
+        Q[3][b] = 2. * RSQ2 * m[4];
+        Q[4][b] = 2. * RSQ2 * m[5];
     }
 }
+
+// Compiler-only barrier: forces loads of loop-invariant tables (material
+// constants, per-orientation matrices) to be reissued after it instead of
+// being hoisted into registers for the whole Newton loop.  Those loads hit
+// the scalar cache or L1; holding them would cost occupancy.
+#define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
 
 __device__ __forceinline__ double ipow(double x, int n)
 {
@@ -578,30 +595,31 @@ __device__ __forceinline__ double ipow(double x, int n)
     return r;
 }
 
-// StrainRate_eval, constitutive.c:4-72.  D holds the symmetric 5x5
-// d(edot)/d(sigma) packed row-major upper (15 entries).
-struct FlowOut {
-    double edot[5];
-    double D[15];
-};
-
-__device__ __forceinline__ void flow_rule(const double sig[6], const double P[12][5], const double *crf,
-                                          const double *crb, long V, long v, const PhaseConst *pc, int nsys,
-                                          FlowOut &o, double *gd, double *tx)
+// StrainRate_eval (constitutive.c:4-72) evaluated in the crystal frame:
+// tau_s = p_s . sx with sx = Q^T sigma; edot = Q ex, D = Q Dx Q^T with
+// ex = sum p_s gamdot_s and Dx = sum p_s p_s^T tau_x_s.  ed and D (packed
+// upper 5x5) are returned in the sample frame.  gd / tx receive gamdot and
+// tau_x when non-null (12 entries each).
+__device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[5][5], const PhaseConst *pc, int nsys,
+                                            const double cf[12], const double cb[12], double ed[5],
+                                            double D[15], double *gd, double *tx)
 {
+    double sx[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) o.edot[i] = 0.0;
+    for (int b = 0; b < 5; ++b) sx[b] = Q[0][b] * sig[0] + Q[1][b] * sig[1] + Q[2][b] * sig[2] + Q[3][b] * sig[3] + Q[4][b] * sig[4];
+    double ex[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double Dx[15];
 #pragma unroll
-    for (int i = 0; i < 15; ++i) o.D[i] = 0.0;
+    for (int i = 0; i < 15; ++i) Dx[i] = 0.0;
     const double nsr = pc->nsr, g0 = pc->gam0;
     const int intn = pc->int_n;
     const int ni = (int)nsr;
 #pragma unroll
     for (int s = 0; s < 12; ++s) {
         if (s < nsys) {
-            const double rss0 = P[s][0] * sig[0] + P[s][1] * sig[1] + P[s][2] * sig[2] + P[s][3] * sig[3] +
-                                P[s][4] * sig[4];
-            const double tc = (rss0 < 0) ? crb[s * V + v] : crf[s * V + v];
+            const double *p = pc->px[s];
+            const double rss0 = p[0] * sx[0] + p[1] * sx[1] + p[2] * sx[2] + p[3] * sx[3] + p[4] * sx[4];
+            const double tc = (rss0 < 0) ? cb[s] : cf[s];
             const double rr = rss0 / tc;
             double pn1, pn;
             if (intn) {
@@ -616,23 +634,60 @@ __device__ __forceinline__ void flow_rule(const double sig[6], const double P[12
             const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
             if (gd) gd[s] = rss2;
             if (tx) tx[s] = rss1;
+            double t[5];
 #pragma unroll
-            for (int i = 0; i < 5; ++i) o.edot[i] += P[s][i] * rss2;
+            for (int a = 0; a < 5; ++a) {
+                ex[a] += p[a] * rss2;
+                t[a] = p[a] * rss1;
+            }
             int q = 0;
 #pragma unroll
-            for (int i = 0; i < 5; ++i)
+            for (int a = 0; a < 5; ++a)
 #pragma unroll
-                for (int j = i; j < 5; ++j) { o.D[q] += P[s][i] * P[s][j] * rss1; ++q; }
+                for (int b = a; b < 5; ++b) { Dx[q] += t[a] * p[b]; ++q; }
         }
     }
+#pragma unroll
+    for (int a = 0; a < 5; ++a) ed[a] = Q[a][0] * ex[0] + Q[a][1] * ex[1] + Q[a][2] * ex[2] + Q[a][3] * ex[3] + Q[a][4] * ex[4];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) D[i] = 0.0;
+    // D = Q Dx Q^T, accumulated one column of Dx Q^T at a time
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        double m[5];
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            double acc = 0.0;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                const int lo = c < e ? c : e, hi = c < e ? e : c;
+                acc += Dx[lo * 5 - (lo * (lo - 1)) / 2 + (hi - lo)] * Q[b][e];
+            }
+            m[b] = acc;
+        }
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int b = a; b < 5; ++b) { D[q] += Q[a][c] * m[b]; ++q; }
+    }
+}
+
+// Same, with the basis rotation rebuilt from R (keeps 9 values live
+// across the Newton loop instead of 25).
+__device__ __forceinline__ void flow_rule(const double sig[6], const double R[9], const PhaseConst *pc, int nsys,
+                                          const double cf[12], const double cb[12], double ed[5], double D[15],
+                                          double *gd, double *tx)
+{
+    double Q[5][5];
+    basis_rotation(R, Q);
+    flow_rule_q(sig, Q, pc, nsys, cf, cb, ed, D, gd, tx);
 }
 
 __device__ __forceinline__ double Dget(const double D[15], int i, int j)
 {
-    // compile-time i, j after unrolling
     const int a = i < j ? i : j, b = i < j ? j : i;
-    const int q = a * 5 - (a * (a - 1)) / 2 + (b - a);
-    return D[q];
+    return D[a * 5 - (a * (a - 1)) / 2 + (b - a)];
 }
 
 // Crout LU with implicit row scaling and partial pivoting
@@ -691,7 +746,6 @@ __device__ __forceinline__ int lu_solve6(double a[6][6], double b[6])
             for (int i = j + 1; i < 6; ++i) a[i][j] *= dum;
         }
     }
-    // forward (unit lower) and back substitution
 #pragma unroll
     for (int i = 1; i < 6; ++i) {
         double sum = b[i];
@@ -705,6 +759,163 @@ __device__ __forceinline__ int lu_solve6(double a[6][6], double b[6])
 #pragma unroll
         for (int k = i + 1; k < 6; ++k) sum -= a[i][k] * b[k];
         b[i] = sum / a[i][i];
+    }
+    return 0;
+}
+
+// The same factorisation and solve, organised for a wavefront: the pivot
+// criterion |a_ij| / max_k |a_ik| (implicit scaling) is compared by cross
+// multiplication instead of through reciprocals, row exchanges run only if
+// some lane of the wave needs one (the Newton Jacobian I + C0 (S + dt D) is
+// diagonally dominant, so in practice none does), and the back substitution
+// reuses the pivot reciprocals.  Same pivots as lu_solve6 except on exact
+// ties of the rounded criterion.
+__device__ __forceinline__ int lu_solve6_wave(double a[6][6], double b[6])
+{
+    double big[6];
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double m = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) m = fmax(m, fabs(a[i][j]));
+        big[i] = m;
+        bad |= (m < 1E-5);
+    }
+    if (bad) return 35;
+    double inv[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+        for (int i = 0; i < j; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+        }
+        double bn = 0.0, bd = 1.0;
+        int imax = j;
+#pragma unroll
+        for (int i = j; i < 6; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+            // |sum| / big_i >= bn / bd
+            if (fabs(sum) * bd >= bn * big[i]) { bn = fabs(sum); bd = big[i]; imax = i; }
+        }
+        if (__any(imax != j)) {
+#pragma unroll
+            for (int i = j + 1; i < 6; ++i) {
+                if (imax == i) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double t = a[i][k];
+                        a[i][k] = a[j][k];
+                        a[j][k] = t;
+                    }
+                    big[i] = big[j];
+                    const double tb = b[i];
+                    b[i] = b[j];
+                    b[j] = tb;
+                }
+            }
+        }
+        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
+        inv[j] = 1.0 / a[j][j];
+        if (j != 5) {
+#pragma unroll
+            for (int i = j + 1; i < 6; ++i) a[i][j] *= inv[j];
+        }
+    }
+#pragma unroll
+    for (int i = 1; i < 6; ++i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum * inv[i];
+    }
+    return 0;
+}
+
+// 5x5 version of lu_solve6_wave (same pivot rule and singular-row check).
+__device__ __forceinline__ int lu_solve5_wave(double a[5][5], double b[6])
+{
+    double big[5];
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double m = 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) m = fmax(m, fabs(a[i][j]));
+        big[i] = m;
+        bad |= (m < 1E-5);
+    }
+    if (bad) return 35;
+    double inv[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+#pragma unroll
+        for (int i = 0; i < j; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+        }
+        double bn = 0.0, bd = 1.0;
+        int imax = j;
+#pragma unroll
+        for (int i = j; i < 5; ++i) {
+            double sum = a[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
+            a[i][j] = sum;
+            if (fabs(sum) * bd >= bn * big[i]) { bn = fabs(sum); bd = big[i]; imax = i; }
+        }
+        if (__any(imax != j)) {
+#pragma unroll
+            for (int i = j + 1; i < 5; ++i) {
+                if (imax == i) {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        const double t = a[i][k];
+                        a[i][k] = a[j][k];
+                        a[j][k] = t;
+                    }
+                    big[i] = big[j];
+                    const double tb = b[i];
+                    b[i] = b[j];
+                    b[j] = tb;
+                }
+            }
+        }
+        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
+        inv[j] = 1.0 / a[j][j];
+        if (j != 4) {
+#pragma unroll
+            for (int i = j + 1; i < 5; ++i) a[i][j] *= inv[j];
+        }
+    }
+#pragma unroll
+    for (int i = 1; i < 5; ++i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum;
+    }
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+        double sum = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 5; ++k) sum -= a[i][k] * b[k];
+        b[i] = sum * inv[i];
     }
     return 0;
 }
@@ -733,126 +944,37 @@ __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *o
     }
 }
 
-// ------------------------------------------------------------------
-// Newton-Raphson stress update, UpdateStress (evolution.c:3-168).
-// Partials per block: [0] sum|sigma-lambda|, [1] sum|eps_tot-eps|,
-// [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1.
-// ------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(NT) void k_nr(RveDev r)
+// Fixed-order sum of nb partial vectors (NV values each, stride 16) by one
+// block; the result lands in tot[] (shared).  Reads bypass L1 so that the
+// partials other blocks wrote in this launch are seen.
+template <int NT, int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int nb, double *tot)
 {
-    const MacroState *ms = r.ms;
-    if (!iter_active(ms)) return;
-    const long V = r.V;
-    const long v = (long)blockIdx.x * NT + threadIdx.x;
-    double acc[14];
+    __shared__ double red[NT / 64][NV];
+    double a[NV];
 #pragma unroll
-    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    if (v < V) {
-        const uint32_t ori = r.ORI[v];
-        const int ph = r.PH[v];
-        if (ori == VOCE_GAS) {
+    for (int i = 0; i < NV; ++i) a[i] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT)
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                r.S6[i * V + v] = 0.0;
-                r.P6[i * V + v] = 0.0;
-            }
-        } else {
-            const PhaseConst *pc = r.phase + (ph - 1);
-            const int nsys = pc->nsys;
-            const double dt = ms->dt;
-            double strain6[6], lam[6], eps6[6], sig[6];
+        for (int i = 0; i < NV; ++i) a[i] += __builtin_nontemporal_load(part + (long)b * 16 + i);
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                strain6[i] = r.E6[i * V + v];
-                lam[i] = r.S6[i * V + v];
-                sig[i] = lam[i];
-            }
+    for (int i = 0; i < NV; ++i) {
+        double x = a[i];
 #pragma unroll
-            for (int i = 0; i < 5; ++i) eps6[i] = r.EP5[i * V + v];
-            eps6[5] = 0.0;
-            double Rm[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
-            double P[12][5];
-            schmid_sample(Rm, pc, nsys, P);
-            const double *crf = r.CRSS;
-            const double *crb = pc->two_crss ? r.CRSSB : r.CRSS;
-            const double *Sl = r.SLOC + (long)ori * 36;
-            const double *C0 = ms->C066;
-            double signorm = 0.0;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) signorm += lam[i] * lam[i];
-            signorm = sqrt(signorm);
-            double erral = 1E-6;   // 10 * erroral
-            double conv_e = 0.0, conv_s = 0.0;
-            double sig_old[6];
-            int iterl = 0;
-            while ((iterl < 100) && (fabs(erral) > 1E-7)) {
-                ++iterl;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) sig_old[i] = sig[i];
-                FlowOut fo;
-                flow_rule(sig, P, crf, crb, V, v, pc, nsys, fo, nullptr, nullptr);
-                double tot[6], res[6];
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double tacc = eps6[i] + (i < 5 ? fo.edot[i] : 0.0) * dt;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) tacc += Sl[i * 6 + j] * sig[j];
-                    tot[i] = tacc;
-                }
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double racc = sig[i] - lam[i];
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) racc += C0[i * 6 + j] * (tot[j] - strain6[j]);
-                    res[i] = racc;
-                }
-                double J[6][6];
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        double jacc = (double)(i == j);
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            const double dk = (k < 5 && j < 5) ? Dget(fo.D, k, j) : 0.0;
-                            jacc += C0[i * 6 + k] * (Sl[k * 6 + j] + dk * dt);
-                        }
-                        J[i][j] = jacc;
-                    }
-                if (lu_solve6(J, res)) {
-                    r.ms->error = 35;
-                    break;
-                }
-                double cn = 0.0, ce = 0.0, cs = 0.0;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    sig[i] -= res[i];
-                    cn += (sig[i] - sig_old[i]) * (sig[i] - sig_old[i]);
-                    ce += (tot[i] - strain6[i]) * (tot[i] - strain6[i]);
-                    cs += (sig[i] - lam[i]) * (sig[i] - lam[i]);
-                }
-                erral = sqrt(cn) / signorm;
-                conv_e = sqrt(ce);
-                conv_s = sqrt(cs);
-            }
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                r.S6[i * V + v] = sig[i];
-                r.P6[i * V + v] = sig_old[i];
-            }
-            acc[0] = conv_s;
-            acc[1] = conv_e;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                acc[2 + i] = sig[i];
-                acc[8 + i] = (ph == 1) ? sig[i] : 0.0;
-            }
-        }
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        a[i] = x;
     }
-    block_reduce_store<NT, 14>(acc, r.part + (long)blockIdx.x * 16);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[threadIdx.x >> 6][i] = a[i];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
+        tot[threadIdx.x] = acc;
+    }
+    __syncthreads();
 }
 
 // tensor of a B-basis 6-vector (chg_basis opt 1)
@@ -907,96 +1029,378 @@ __device__ double svm_of(const double S[9])
     return sqrt(3. / 2. * v);
 }
 
-// ------------------------------------------------------------------
-// Single-block finaliser of one iteration (evolution.c:405-428, 170-241)
-// ------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(NT) void k_fin_iter(RveDev r)
+// Finaliser of one iteration (evolution.c:405-428 and get_smacro,
+// evolution.c:170-241), run by thread 0 on the summed partials.
+__device__ void finalize_iteration(MacroState *ms, const double *tot)
 {
-    MacroState *ms = r.ms;
-    if (!iter_active(ms)) return;
-    __shared__ double tot[14];
-    __shared__ double red[NT / 64][14];
-    const int nb = r.nblk_vox;
-    double a[14];
-#pragma unroll
-    for (int i = 0; i < 14; ++i) a[i] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += NT)
-#pragma unroll
-        for (int i = 0; i < 14; ++i) a[i] += r.part[(long)b * 16 + i];
-#pragma unroll
-    for (int i = 0; i < 14; ++i) {
-        double x = a[i];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        a[i] = x;
+    // all reads first, then the arithmetic, then all writes: loads and
+    // stores share one wait counter, so interleaving them would serialise
+    // every access
+    const double wgt = ms->wgt, wgt1 = ms->wgt_over_ph1, e_vm = ms->e_vm, err_tol = ms->err_tol;
+    const int it = ms->iter, itot = ms->iter_total, itmax = ms->iter_max;
+    int vfl[6], sfl[6];
+    double Sc[9], acum[9], applied[9], S0s[6][6];
+    const int IJV0[6] = {0, 1, 2, 1, 0, 0}, IJV1[6] = {0, 1, 2, 2, 2, 1};
+    for (int i = 0; i < 6; ++i) {
+        vfl[i] = ms->velgrad_flag[i];
+        sfl[i] = ms->stress_flag[i];
+        for (int k = 0; k < 6; ++k)
+            S0s[i][k] = ms->S0[((IJV0[i] * 3 + IJV1[i]) * 3 + IJV0[k]) * 3 + IJV1[k]];
     }
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int i = 0; i < 14; ++i) red[threadIdx.x >> 6][i] = a[i];
-    __syncthreads();
-    if (threadIdx.x < 14) {
-        double acc = 0.0;
-        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
-        tot[threadIdx.x] = acc;
+    for (int i = 0; i < 9; ++i) {
+        Sc[i] = ms->Scauchy[i];
+        acum[i] = ms->acum[i];
+        applied[i] = ms->applied[i];
     }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    // SigAvg, SigAvg1 (evolution.c:182-197)
     double s6[6], s16[6], SA[9], SA1[9];
     for (int i = 0; i < 6; ++i) { s6[i] = tot[2 + i]; s16[i] = tot[8 + i]; }
     b6_to_t(s6, SA);
     b6_to_t(s16, SA1);
     for (int i = 0; i < 9; ++i) {
-        SA[i] *= ms->wgt;
-        SA1[i] *= ms->wgt_over_ph1;
-        ms->SigAvg[i] = SA[i];
-        ms->SigAvg1[i] = SA1[i];
+        SA[i] *= wgt;
+        SA1[i] *= wgt1;
     }
     // mixed-BC correction, upper triangle only (evolution.c:199-214)
-    const int IJV0[6] = {0, 1, 2, 1, 0, 0}, IJV1[6] = {0, 1, 2, 2, 2, 1};
     double dD[9];
     for (int i = 0; i < 9; ++i) dD[i] = 0.0;
     for (int i = 0; i < 6; ++i) {
-        const int ii = IJV0[i], jj = IJV1[i];
         double acc = 0.0;
-        if (ms->velgrad_flag[i] == 0) {
+        if (vfl[i] == 0) {
             for (int k = 0; k < 6; ++k) {
                 const int kk = IJV0[k], ll = IJV1[k];
-                acc += ms->S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * ms->stress_flag[k] *
-                       (ms->Scauchy[kk * 3 + ll] - SA[kk * 3 + ll]);
+                acc += S0s[i][k] * sfl[k] * (Sc[kk * 3 + ll] - SA[kk * 3 + ll]);
             }
         }
-        dD[ii * 3 + jj] = acc;
+        dD[IJV0[i] * 3 + IJV1[i]] = acc;
+    }
+    for (int i = 0; i < 9; ++i) acum[i] += dD[i];
+    const double s_vm = svm_of(SA), s_vm1 = svm_of(SA1);
+    double Err_e = tot[1] * wgt, Err_s = tot[0] * wgt;
+    if (fabs(e_vm) > 1E-5) Err_e /= e_vm;
+    Err_s /= s_vm;
+    const int more = (it + 1 < itmax) && (fmax(fabs(Err_s), fabs(Err_e)) > fabs(err_tol));
+    double sh[6];
+    if (more) {
+        for (int i = 0; i < 9; ++i) applied[i] += dD[i];
+        sym_to_b6(dD, sh);
     }
     for (int i = 0; i < 9; ++i) {
+        ms->SigAvg[i] = SA[i];
+        ms->SigAvg1[i] = SA1[i];
         ms->dD[i] = dD[i];
-        ms->acum[i] += dD[i];
+        ms->acum[i] = acum[i];
     }
-    ms->s_vm = svm_of(SA);
-    ms->s_vm1 = svm_of(SA1);
-    double Err_e = tot[1] * ms->wgt, Err_s = tot[0] * ms->wgt;
-    if (fabs(ms->e_vm) > 1E-5) Err_e /= ms->e_vm;
-    Err_s /= ms->s_vm;
+    ms->s_vm = s_vm;
+    ms->s_vm1 = s_vm1;
     ms->Err_e = Err_e;
     ms->Err_s = Err_s;
-    const int it = ms->iter;
     if (it < VOCE_MAXIT_LOG) {
         ms->log_err_e[it] = Err_e;
         ms->log_err_s[it] = Err_s;
-        ms->log_svm[it] = ms->s_vm;
+        ms->log_svm[it] = s_vm;
     }
     ms->iter = it + 1;
-    ms->iter_total += 1;
-    const int more = (it + 1 < ms->iter_max) && (fmax(fabs(Err_s), fabs(Err_e)) > fabs(ms->err_tol));
+    ms->iter_total = itot + 1;
     if (more) {
-        for (int i = 0; i < 9; ++i) ms->applied[i] += dD[i];
-        double sh[6];
-        sym_to_b6(dD, sh);
+        for (int i = 0; i < 9; ++i) ms->applied[i] = applied[i];
         for (int i = 0; i < 6; ++i) ms->shift6[i] = sh[i];
     }
     __threadfence();
     ms->active = more;
+}
+
+// ------------------------------------------------------------------
+// Newton-Raphson stress update, UpdateStress (evolution.c:3-168).
+//
+// Per NR iteration (evolution.c:63-114) the residual and Jacobian of
+// evolution.c:70-94 are, regrouped,
+//   R = A sigma + dt C0 edot - c,   J = A + dt C0 d(edot)/d(sigma)
+// with A = I + C0 S_local (per orientation) and c = lambda + C0 (eps - eps_p);
+// the step J^-1 R is taken as M^-1 A^-1 R with M = I + dt A^-1 C0 D (a 5x5
+// solve, see below), A^-1 and A^-1 C0 coming from the orientation table.
+// StrainRate_eval's sums (constitutive.c:49-64) enter one slip system at a
+// time from the rotated slip vectors n' = R n, b' = R b: the resolved shear
+// n' . sigma b', the Schmid vector P_s = B(sym(n' (x) b')), edot += P_s
+// gamdot_s and D += tau_x_s P_s (x) P_s, so the rotated Schmid set is never
+// held whole.
+//
+// Partials per block: [0] sum|sigma-lambda|, [1] sum|eps_tot-eps|,
+// [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1.  The last
+// block to finish sums all partials in block order and runs the iteration
+// finaliser, so one launch covers UpdateStress, the error Allreduces and
+// get_smacro.  UNI: a single crystalline phase (uniform material constants).
+// ------------------------------------------------------------------
+
+// B-basis (b1..b5) components of R P R^T for a symmetric crystal-frame
+// tensor P given as (P00, P11, P22, P12, P02, P01)
+__device__ __forceinline__ void rotate_sym_b5(const double R[9], const double *p, double out[5])
+{
+    // M = R P (3x3), then only the needed entries of M R^T
+    double M[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double r0 = R[i * 3 + 0], r1 = R[i * 3 + 1], r2 = R[i * 3 + 2];
+        M[i * 3 + 0] = r0 * p[0] + r1 * p[5] + r2 * p[4];
+        M[i * 3 + 1] = r0 * p[5] + r1 * p[1] + r2 * p[3];
+        M[i * 3 + 2] = r0 * p[4] + r1 * p[3] + r2 * p[2];
+    }
+    const double q00 = M[0] * R[0] + M[1] * R[1] + M[2] * R[2];
+    const double q11 = M[3] * R[3] + M[4] * R[4] + M[5] * R[5];
+    const double q22 = M[6] * R[6] + M[7] * R[7] + M[8] * R[8];
+    const double q12 = M[3] * R[6] + M[4] * R[7] + M[5] * R[8];
+    const double q02 = M[0] * R[6] + M[1] * R[7] + M[2] * R[8];
+    const double q01 = M[0] * R[3] + M[1] * R[4] + M[2] * R[5];
+    out[0] = RSQ2 * (q11 - q00);
+    out[1] = RSQ6 * (2. * q22 - q00 - q11);
+    out[2] = 2. * RSQ2 * q12;
+    out[3] = 2. * RSQ2 * q02;
+    out[4] = 2. * RSQ2 * q01;
+}
+
+template <int NT, bool UNI, bool TWO>
+__global__ __launch_bounds__(NT) void k_nr(RveDev r)
+{
+    MacroState *ms = r.ms;
+    if (!iter_active(ms)) return;
+    // per-lane values that stay constant through the Newton loop live in LDS
+    // (component-major: conflict-free), keeping the register file for the loop
+    __shared__ double lds_w[6][NT];     // A^-1 c
+    __shared__ double lds_old[6][NT];   // stress at the last flow-rule evaluation
+    __shared__ double lds_R[9][NT];     // TranMat of the voxel
+    __shared__ double lds_cr[TWO ? 24 : 12][NT];   // forward (and backward) CRSS
+    const long V = r.V;
+    const long v = (long)blockIdx.x * NT + threadIdx.x;
+    double acc[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    if (v < V) {
+        const uint32_t ori = r.ORI[v];
+        const int ph = r.PH[v];
+        if (ori == VOCE_GAS) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                r.S6[i * V + v] = 0.0;
+                r.P6[i * V + v] = 0.0;
+            }
+        } else {
+            const PhaseConst *pc = r.phase + (UNI ? 0 : (ph - 1));
+            const int nsys = pc->nsys;
+            const double dt = ms->dt;
+            const double *C0 = ms->C066;
+            const double *tab = r.SLOC + (long)ori * VOCE_OTAB;   // S_local, A^-1, A^-1 C0
+#pragma unroll
+            for (int i = 0; i < 9; ++i) lds_R[i][threadIdx.x] = r.R[i * V + v];
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                if (q < nsys) {
+                    lds_cr[q][threadIdx.x] = r.CRSS[q * V + v];
+                    if (TWO) lds_cr[TWO ? 12 + q : q][threadIdx.x] = r.CRSSB[q * V + v];
+                }
+            }
+            double sig[6], signorm = 0.0;
+            {
+                double de[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    sig[i] = r.S6[i * V + v];
+                    de[i] = r.E6[i * V + v] - (i < 5 ? r.EP5[i * V + v] : 0.0);
+                    signorm += sig[i] * sig[i];
+                }
+                double c[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double t = sig[i];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) t += C0[i * 6 + j] * de[j];
+                    c[i] = t;
+                }
+                // w = A^-1 c, kept in LDS (a per-lane register extension)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) t += tab[36 + i * 6 + j] * c[j];
+                    lds_w[i][threadIdx.x] = t;
+                }
+            }
+            signorm = sqrt(signorm);
+            const double nsr = pc->nsr, g0 = pc->gam0;
+            const int intn = pc->int_n, ni = (int)nsr;
+            double erral = 1E-6;   // 10 * erroral (evolution.c:58-61)
+            double ed[5];
+            int iterl = 0;
+            while ((iterl < 100) && (fabs(erral) > 1E-7)) {
+                ++iterl;
+                RELOAD_BARRIER();
+#pragma unroll
+                for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
+                double Rm[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Rm[i] = lds_R[i][threadIdx.x];
+                double T[9];
+                b6_to_t(sig, T);
+                // StrainRate_eval (constitutive.c:4-72): edot and the
+                // symmetric 5x5 d(edot)/d(sigma), one slip system at a time.
+                // With n' = R n and b' = R b the sample-frame Schmid tensor is
+                // sym(n' (x) b') (update_schmid, kinematics.c:426-459) and the
+                // resolved shear is n' . sigma b'.
+                double D[15];
+#pragma unroll
+                for (int a = 0; a < 5; ++a) ed[a] = 0.0;
+#pragma unroll
+                for (int q = 0; q < 15; ++q) D[q] = 0.0;
+                // slip vectors of system s + 1 are loaded while s is computed
+                double bnn[3] = {pc->bn[0][0], pc->bn[0][1], pc->bn[0][2]};
+                double bbn[3] = {pc->bb[0][0], pc->bb[0][1], pc->bb[0][2]};
+#pragma unroll 1
+                for (int s = 0; s < nsys; ++s) {
+                    RELOAD_BARRIER();   // slip vectors are re-read from the scalar cache
+                    const double cfs = lds_cr[s][threadIdx.x];
+                    const double cbs = TWO ? lds_cr[TWO ? 12 + s : s][threadIdx.x] : cfs;
+                    const double bn[3] = {bnn[0], bnn[1], bnn[2]}, bb[3] = {bbn[0], bbn[1], bbn[2]};
+                    if (s + 1 < nsys) {
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            bnn[i] = pc->bn[s + 1][i];
+                            bbn[i] = pc->bb[s + 1][i];
+                        }
+                    }
+                    double n[3], bv[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        n[i] = Rm[i * 3 + 0] * bn[0] + Rm[i * 3 + 1] * bn[1] + Rm[i * 3 + 2] * bn[2];
+                        bv[i] = Rm[i * 3 + 0] * bb[0] + Rm[i * 3 + 1] * bb[1] + Rm[i * 3 + 2] * bb[2];
+                    }
+                    double rss0 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        rss0 += n[i] * (T[i * 3 + 0] * bv[0] + T[i * 3 + 1] * bv[1] + T[i * 3 + 2] * bv[2]);
+                    const double tc = (rss0 < 0) ? cbs : cfs;
+                    const double itc = 1.0 / tc;
+                    const double rr = rss0 * itc;
+                    double pn1, pn;
+                    if (intn) {
+                        const double ar = fabs(rr);
+                        pn1 = ipow(ar, ni - 1);
+                        pn = pn1 * ar;
+                    } else {
+                        pn1 = fabs(pow(rr, nsr - 1.));
+                        pn = fabs(pow(rr, nsr));
+                    }
+                    const double rss1 = g0 * nsr * pn1 * itc;
+                    const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+                    // B-basis components of sym(n' (x) b')
+                    double P[5];
+                    {
+                        const double d0 = n[0] * bv[0], d1 = n[1] * bv[1], d2 = n[2] * bv[2];
+                        P[0] = RSQ2 * (d1 - d0);
+                        P[1] = RSQ6 * (2. * d2 - d0 - d1);
+                        P[2] = RSQ2 * (n[1] * bv[2] + n[2] * bv[1]);
+                        P[3] = RSQ2 * (n[0] * bv[2] + n[2] * bv[0]);
+                        P[4] = RSQ2 * (n[0] * bv[1] + n[1] * bv[0]);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) ed[a] += P[a] * rss2;
+                    int q = 0;
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) {
+                        const double t = P[a] * rss1;
+#pragma unroll
+                        for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
+                    }
+                }
+                RELOAD_BARRIER();
+                // Newton step x = J^-1 R with J = A M, M = I + dt (A^-1 C0) D:
+                // z = A^-1 R = sigma - w + dt (A^-1 C0) edot, M x = z.  M is the
+                // identity in its 6th column, so rows 0..4 are a 5x5 system and
+                // x_5 follows from row 5.
+                const double *Bm = tab + 72;   // A^-1 C0
+                double z[6], M[5][5];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double t = sig[i] - lds_w[i][threadIdx.x];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) t += dt * Bm[i * 6 + k] * ed[k];
+                    z[i] = t;
+                }
+#pragma unroll
+                for (int i = 0; i < 5; ++i)
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) t += Bm[i * 6 + k] * Dget(D, k, j);
+                        M[i][j] = (double)(i == j) + dt * t;
+                    }
+                double x[6];
+                if (lu_solve5_wave(M, z)) {
+                    ms->error = 35;
+                    break;
+                }
+                {
+                    double t5 = z[5];
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) t += Bm[5 * 6 + k] * Dget(D, k, j);
+                        t5 -= dt * t * z[j];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) x[i] = z[i];
+                    x[5] = t5;
+                }
+                double cn = 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    const double so = lds_old[i][threadIdx.x];
+                    sig[i] -= x[i];
+                    cn += (sig[i] - so) * (sig[i] - so);
+                }
+                erral = sqrt(cn) / signorm;
+            }
+            // error partials of the last NR iteration (evolution.c:103-114):
+            // |sigma - lambda| and |eps_tot(sigma_old) - eps|
+            double cs = 0.0, ce = 0.0, sig_old[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sig_old[i] = lds_old[i][threadIdx.x];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const double lam = r.S6[i * V + v];
+                cs += (sig[i] - lam) * (sig[i] - lam);
+                double t = (i < 5 ? r.EP5[i * V + v] + ed[i] * dt : 0.0) - r.E6[i * V + v];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) t += tab[i * 6 + j] * sig_old[j];
+                ce += t * t;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                r.S6[i * V + v] = sig[i];
+                r.P6[i * V + v] = sig_old[i];
+            }
+            acc[0] = sqrt(cs);
+            acc[1] = sqrt(ce);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                acc[2 + i] = sig[i];
+                acc[8 + i] = (ph == 1) ? sig[i] : 0.0;
+            }
+        }
+    }
+    block_reduce_store<NT, 14>(acc, r.part + (long)blockIdx.x * 16);
+    // last block finalises the iteration
+    __shared__ int amlast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) amlast = (atomicAdd(&ms->ticket, 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (!amlast) return;
+    __shared__ double tot[16];
+    sum_partials<NT, 14>(r.part, gridDim.x, tot);
+    if (threadIdx.x == 0) {
+        ms->ticket = 0;
+        finalize_iteration(ms, tot);
+    }
 }
 
 // increment start (evolution.c:306-340)
@@ -1005,6 +1409,7 @@ __global__ void k_begin_increment(RveDev r)
     MacroState *ms = r.ms;
     ms->istep += 1;
     ms->iter = 0;
+    ms->ticket = 0;
     for (int i = 0; i < 9; ++i) {
         ms->dD[i] = 0.0;
         ms->acum[i] = 0.0;
@@ -1062,7 +1467,7 @@ __device__ void orientation(double tran[9], const double rot_in[9])
 // Increment close-out per voxel (evolution.c:433-530).  Partials per
 // block: [0..4] sum Eps (B basis), [5..9] sum Edot.
 // ------------------------------------------------------------------
-template <int NT>
+template <int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
 {
     const MacroState *ms = r.ms;
@@ -1084,6 +1489,7 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
             const double sa[3] = {(A[7] - A[5]) / 2., (A[2] - A[6]) / 2., (A[3] - A[1]) / 2.};
             double base[3];
             if (ms->creep) {
+#pragma unroll
                 for (int i = 0; i < 3; ++i) base[i] = ms->creep_skew[i] - r.WG[i * V + v];
             } else {
                 const double *U = ms->Udot;
@@ -1091,6 +1497,7 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
                 base[1] = (U[2] - U[6]) / 2. * dt;
                 base[2] = (U[3] - U[1]) / 2. * dt;
             }
+#pragma unroll
             for (int i = 0; i < 3; ++i) {
                 dsk[i] = base[i] + sa[i] + r.WF[i * V + v];
                 r.WG[i * V + v] += dsk[i];
@@ -1098,30 +1505,44 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
             }
         }
         if (ori == VOCE_GAS) {
+#pragma unroll
             for (int i = 0; i < 5; ++i) r.ED5[i * V + v] = 0.0;
+#pragma unroll
             for (int i = 0; i < 5; ++i) acc[i] = r.EP5[i * V + v];
         } else {
-            const PhaseConst *pc = r.phase + (ph - 1);
+            const PhaseConst *pc = r.phase + (UNI ? 0 : (ph - 1));
             const int nsys = pc->nsys;
             double Rm[9];
+#pragma unroll
             for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
-            double P[12][5];
-            schmid_sample(Rm, pc, nsys, P);
+            double Q[5][5];
+            basis_rotation(Rm, Q);
             double sp[6];
+#pragma unroll
             for (int i = 0; i < 6; ++i) sp[i] = r.P6[i * V + v];
             double *crf = r.CRSS;
             double *crb = pc->two_crss ? r.CRSSB : r.CRSS;
-            FlowOut fo;
-            double gd[12], tx[12];
-            flow_rule(sp, P, crf, crb, V, v, pc, nsys, fo, gd, tx);
+            double cf[12], cb[12];
+#pragma unroll
+            for (int s = 0; s < 12; ++s) {
+                cf[s] = s < nsys ? crf[s * V + v] : 1.0;
+                cb[s] = s < nsys ? crb[s * V + v] : 1.0;
+            }
+            double ed[5], D[15], gd[12], tx[12];
+#pragma unroll
+            for (int s = 0; s < 12; ++s) gd[s] = tx[s] = 0.0;
+            flow_rule_q(sp, Q, pc, nsys, cf, cb, ed, D, gd, tx);
             double ep[5];
+#pragma unroll
             for (int i = 0; i < 5; ++i) ep[i] = r.EP5[i * V + v];
             if (diag) {
                 // EVM from eps_tot at the pre-update stress, SVM from sigma (evolution.c:134-155)
-                const double *Sl = r.SLOC + (long)ori * 36;
+                const double *Sl = r.SLOC + (long)ori * VOCE_OTAB;
                 double tt[6];
+#pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    double t = (i < 5 ? ep[i] : 0.0) + (i < 5 ? fo.edot[i] : 0.0) * dt;
+                    double t = (i < 5 ? ep[i] + ed[i] * dt : 0.0);
+#pragma unroll
                     for (int j = 0; j < 6; ++j) t += Sl[i * 6 + j] * sp[j];
                     tt[i] = t;
                 }
@@ -1129,57 +1550,74 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
                 b6_to_t(tt, T);
                 r.EVM[v] = von_mises_strain(T);
                 double s6[6];
+#pragma unroll
                 for (int i = 0; i < 6; ++i) s6[i] = r.S6[i * V + v];
                 double ST[9];
                 b6_to_t(s6, ST);
                 const double tr = ST[0] + ST[4] + ST[8];
                 double sv = 0.0;
+#pragma unroll
                 for (int i = 0; i < 3; ++i)
+#pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const double d = (ST[i * 3 + j] + ST[j * 3 + i]) / 2. - (double)(i == j) * tr / 3.0;
                         sv += d * d;
                     }
                 r.SVM[v] = sqrt(3. / 2. * sv);
+#pragma unroll
                 for (int s = 0; s < 12; ++s) {
-                    r.GAMDOT[s * V + v] = s < nsys ? gd[s] : 0.0;
-                    r.TAUX[s * V + v] = s < nsys ? tx[s] : 0.0;
+                    r.GAMDOT[s * V + v] = gd[s];
+                    r.TAUX[s * V + v] = tx[s];
                 }
             }
+#pragma unroll
             for (int i = 0; i < 5; ++i) {
-                ep[i] += fo.edot[i] * dt;
+                ep[i] += ed[i] * dt;
                 r.EP5[i * V + v] = ep[i];
-                r.ED5[i * V + v] = fo.edot[i];
+                r.ED5[i * V + v] = ed[i];
                 acc[i] = ep[i];
-                acc[5 + i] = fo.edot[i];
+                acc[5 + i] = ed[i];
             }
             // update_orient (constitutive.c:171-248), from step 2
             if (ms->update_flag == 1 && istep > 1) {
                 double Lp[9];
+#pragma unroll
                 for (int i = 0; i < 9; ++i) Lp[i] = 0.0;
-                for (int s = 0; s < nsys; ++s) {
-                    double bsa[3], nsa[3];
-                    for (int i = 0; i < 3; ++i) {
-                        bsa[i] = 0.0;
-                        nsa[i] = 0.0;
-                        for (int j = 0; j < 3; ++j) {
-                            bsa[i] += Rm[i * 3 + j] * pc->bb[s][j];
-                            nsa[i] += Rm[i * 3 + j] * pc->bn[s][j];
+#pragma unroll
+                for (int s = 0; s < 12; ++s) {
+                    if (s < nsys) {
+                        double bsa[3], nsa[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            bsa[i] = 0.0;
+                            nsa[i] = 0.0;
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) {
+                                bsa[i] += Rm[i * 3 + j] * pc->bb[s][j];
+                                nsa[i] += Rm[i * 3 + j] * pc->bn[s][j];
+                            }
                         }
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) Lp[i * 3 + j] += bsa[i] * nsa[j] * gd[s];
                     }
-                    for (int i = 0; i < 3; ++i)
-                        for (int j = 0; j < 3; ++j) Lp[i * 3 + j] += bsa[i] * nsa[j] * gd[s];
                 }
-                double RotLoc[9] = {0.0, -dsk[2] / dt, dsk[1] / dt, dsk[2] / dt, 0.0, -dsk[0] / dt,
-                                    -dsk[1] / dt, dsk[0] / dt, 0.0};
+                const double RotLoc[9] = {0.0, -dsk[2] / dt, dsk[1] / dt, dsk[2] / dt, 0.0, -dsk[0] / dt,
+                                          -dsk[1] / dt, dsk[0] / dt, 0.0};
                 double Rot[9];
+#pragma unroll
                 for (int i = 0; i < 3; ++i)
+#pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const double rs = (Lp[i * 3 + j] - Lp[j * 3 + i]) / 2.;
                         Rot[i * 3 + j] = (ms->Udot_a[i * 3 + j] + RotLoc[i * 3 + j] - rs) * dt;
                     }
                 double Rn[9];
+#pragma unroll
                 for (int i = 0; i < 9; ++i) Rn[i] = Rm[i];
                 orientation(Rn, Rot);
+#pragma unroll
                 for (int i = 0; i < 9; ++i) r.R[i * V + v] = Rn[i];
             }
             // harden (constitutive.c:250-310), from step 3, and the
@@ -1187,10 +1625,12 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
             if (ms->hard_flag == 1 && istep > 2) {
                 const double gamtot = r.GAC[v];
                 double deltgam = 0.0;
-                for (int s = 0; s < nsys; ++s) {
-                    r.GTOT[s * V + v] = r.GTOT[s * V + v] + gd[s] * dt;
-                }
-                for (int s = 0; s < nsys; ++s) deltgam += fabs(gd[s]) * dt;
+#pragma unroll
+                for (int s = 0; s < 12; ++s)
+                    if (s < nsys) r.GTOT[s * V + v] = r.GTOT[s * V + v] + gd[s] * dt;
+#pragma unroll
+                for (int s = 0; s < 12; ++s)
+                    if (s < nsys) deltgam += fabs(gd[s]) * dt;
                 const double tau0 = pc->tau0f, tau1 = pc->tau1, thet0 = pc->thet0, thet1 = pc->thet1;
                 const double TINY = 1.E-4 * tau0;
                 double voce = 0.0;
@@ -1204,24 +1644,37 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
                                 thet1 / fact * exp_ini * (exp_del * ((gamtot + deltgam) * fact + 1.0) - (gamtot * fact + 1.0));
                     }
                 }
-                for (int s = 0; s < nsys; ++s) {
-                    double dtau = 0.;
-                    for (int j = 0; j < nsys; ++j) dtau += pc->hard[s][j] * fabs(gd[j]) * dt;
-                    crf[s * V + v] += dtau * voce / deltgam;
-                    if (pc->two_crss) crb[s * V + v] += dtau * voce / deltgam;
+#pragma unroll
+                for (int s = 0; s < 12; ++s) {
+                    if (s < nsys) {
+                        double dtau = 0.;
+#pragma unroll
+                        for (int j = 0; j < 12; ++j)
+                            if (j < nsys) dtau += pc->hard[s][j] * fabs(gd[j]) * dt;
+                        crf[s * V + v] += dtau * voce / deltgam;
+                        if (pc->two_crss) crb[s * V + v] += dtau * voce / deltgam;
+                    }
                 }
                 r.GAC[v] = gamtot + deltgam;
                 if (diag) {
                     double s6[6];
+#pragma unroll
                     for (int i = 0; i < 6; ++i) s6[i] = r.S6[i * V + v];
+                    double sx[5];
+#pragma unroll
+                    for (int b = 0; b < 5; ++b)
+                        sx[b] = Q[0][b] * s6[0] + Q[1][b] * s6[1] + Q[2][b] * s6[2] + Q[3][b] * s6[3] + Q[4][b] * s6[4];
                     double rr = 0.0, rc = 0.0, wp = 0.0;
                     const double K = (0.35 * 47.565 * 0.256);
-                    for (int s = 0; s < nsys; ++s) {
-                        const double rss = P[s][0] * s6[0] + P[s][1] * s6[1] + P[s][2] * s6[2] + P[s][3] * s6[3] +
-                                           P[s][4] * s6[4];
-                        rr += (rss / K) * (rss / K);
-                        rc += (crf[s * V + v] / K) * (crf[s * V + v] / K);
-                        wp += fabs((r.GTOT[s * V + v]) * rss);
+#pragma unroll
+                    for (int s = 0; s < 12; ++s) {
+                        if (s < nsys) {
+                            const double *p = pc->px[s];
+                            const double rss = p[0] * sx[0] + p[1] * sx[1] + p[2] * sx[2] + p[3] * sx[3] + p[4] * sx[4];
+                            rr += (rss / K) * (rss / K);
+                            rc += (crf[s * V + v] / K) * (crf[s * V + v] / K);
+                            wp += fabs((r.GTOT[s * V + v]) * rss);
+                        }
                     }
                     r.RHOR[v] = rr;
                     r.RHOC[v] = rc;
@@ -1238,26 +1691,8 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_fin_incr(RveDev r)
 {
     MacroState *ms = r.ms;
-    __shared__ double tot[10];
-    __shared__ double red[NT / 64][10];
-    double a[10];
-    for (int i = 0; i < 10; ++i) a[i] = 0.0;
-    for (int b = threadIdx.x; b < r.nblk_vox; b += NT)
-        for (int i = 0; i < 10; ++i) a[i] += r.part[(long)b * 16 + i];
-    for (int i = 0; i < 10; ++i) {
-        double x = a[i];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        a[i] = x;
-    }
-    if ((threadIdx.x & 63) == 0)
-        for (int i = 0; i < 10; ++i) red[threadIdx.x >> 6][i] = a[i];
-    __syncthreads();
-    if (threadIdx.x < 10) {
-        double acc = 0.0;
-        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
-        tot[threadIdx.x] = acc;
-    }
-    __syncthreads();
+    __shared__ double tot[16];
+    sum_partials<NT, 10>(r.part, r.nblk_vox, tot);
     if (threadIdx.x != 0) return;
     const double dt = ms->dt;
     for (int i = 0; i < 9; ++i) {
@@ -1340,9 +1775,14 @@ hipError_t launch_iter_t(const RveDev *r, hipStream_t st, hipEvent_t *ev)
         hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 6), dim3(NT_PL_S), pl_lds, st, *r, 0);
     }
     MARK();
-    hipLaunchKernelGGL((k_nr<NT_VOX>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+    if (r->uniform_phase) {
+        if (r->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+    } else {
+        if (r->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+    }
     MARK();
-    hipLaunchKernelGGL((k_fin_iter<NT_FIN>), dim3(1), dim3(NT_FIN), 0, st, *r);
     MARK();
 #undef MARK
     return hipGetLastError();
@@ -1414,7 +1854,8 @@ extern "C" int voce_launch_closeout(const RveDev *r, void *stream, int need_skew
     hipError_t e = hipSuccess;
     if (need_skew) e = r->fft_f32 ? launch_skew_t<float>(r, st) : launch_skew_t<double>(r, st);
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((k_closeout<NT_VOX>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
+    if (r->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
+    else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
     hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(1), dim3(NT_FIN), 0, st, *r);
     return (int)hipGetLastError();
 }
