@@ -242,7 +242,9 @@ struct voce_ctx {
     int pred_iters;
     int update_flag, creep, iter_max;
     hipStream_t stream;
-    RveDev d;
+    RveDev d;              // host copy of the descriptor
+    RveDev *dtab;          // device copy (a one-entry RVE table)
+    LaunchSet L;           // launch sequence over that table
     MacroState *h_ms;      // pinned mirror of the device MacroState
     int *h_ctl;            // pinned: istep, iter, iter_total, active, error
     int two_crss_any;
@@ -262,6 +264,28 @@ static int dev_alloc(voce_ctx *c, void **p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return VOCE_ERR_NOMEM;
     c->allocs.push_back(*p);
+    return 0;
+}
+
+// Push the host descriptor to its device copy and refresh the launch set.
+static int sync_tab(voce_ctx *c)
+{
+    const RveDev &d = c->d;
+    if (hipMemcpyAsync(c->dtab, &d, sizeof(RveDev), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return VOCE_ERR_DEVICE;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return VOCE_ERR_DEVICE;
+    LaunchSet &L = c->L;
+    memset(&L, 0, sizeof(L));
+    L.tab = c->dtab;
+    L.n_plane6 = d.nx * 6;
+    L.n_plane3 = d.nx * 3;
+    L.n_x = (d.ny * d.nzh + d.tk - 1) / d.tk;
+    L.n_vox = d.nblk_vox;
+    L.n_one = 1;
+    voce_launch_shape(&d, &L.large, &L.pl_lds, &L.x_lds);
+    L.fft_f32 = d.fft_f32;
+    L.uniform_phase = d.uniform_phase;
+    L.two_crss = d.two_crss;
     return 0;
 }
 
@@ -302,12 +326,13 @@ extern "C" long voce_ctx_nvox(const voce_ctx *c) { return c ? c->V : 0; }
 extern "C" void voce_ctx_destroy(voce_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void *p : c->allocs) hipFree(p);
-    for (hipEvent_t e : c->ev) hipEventDestroy(e);
-    if (c->h_ms) hipHostFree(c->h_ms);
-    if (c->h_ctl) hipHostFree(c->h_ctl);
-    if (c->stream) hipStreamDestroy(c->stream);
+    // teardown: errors here have no caller to go to
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->h_ms) (void)hipHostFree(c->h_ms);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -578,7 +603,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
         ptr = (decltype(ptr))p_;                                      \
     } while (0)
     if (hipStreamCreate(&c->stream) != hipSuccess) { free(ms0); voce_ctx_destroy(c); return VOCE_ERR_DEVICE; }
-    voce_kernels_prepare();
+    if (voce_kernels_prepare() != 0) { free(ms0); voce_ctx_destroy(c); return VOCE_ERR_DEVICE; }
     RveDev &d = c->d;
     memset(&d, 0, sizeof(d));
     d.nx = nx; d.ny = ny; d.nz = nz; d.nzh = nz / 2 + 1;
@@ -588,6 +613,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     // one crystalline phase, numbered 1: its constants are uniform across a wave
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;
+    d.tk = voce_xgreen_tile(nx);
     const size_t cb = d.fft_f32 ? 8 : 16;
     const size_t db = sizeof(double);
     ALLOC(d.E6, 6 * V * db);
@@ -615,6 +641,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     ALLOC(d.tw_x, nx * cb);
     ALLOC(d.tw_y, ny * cb);
     ALLOC(d.tw_z, nz * cb);
+    ALLOC(c->dtab, sizeof(RveDev));
     double *sig6o_d = nullptr;
     ALLOC(sig6o_d, sig6o.size() * db);
     if (hipHostMalloc((void **)&c->h_ms, sizeof(MacroState), hipHostMallocDefault) != hipSuccess ||
@@ -668,7 +695,9 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
             H2D(d.tw_x, tx.data(), nx * cb); H2D(d.tw_y, ty.data(), ny * cb); H2D(d.tw_z, tz.data(), nz * cb);
             CK(hipStreamSynchronize(c->stream));
         }
-        e = voce_launch_init_fields(&d, sig6o_d, c->stream);
+        e = sync_tab(c);
+        if (e) break;
+        e = voce_launch_init_fields(c->dtab, V, sig6o_d, c->stream);
         if (e) break;
         CK(hipStreamSynchronize(c->stream));
     } while (0);
@@ -695,7 +724,7 @@ static int ensure_diag(voce_ctx *c)
         if (hipMemsetAsync(p, 0, w.n, c->stream) != hipSuccess) return VOCE_ERR_DEVICE;
         *w.dst = (double *)p;
     }
-    return 0;
+    return sync_tab(c);
 }
 
 extern "C" int voce_set_diagnostics(voce_ctx *c, int on)
@@ -716,7 +745,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
 {
     if (!c) return VOCE_ERR_ARG;
     RveDev &d = c->d;
-    int e = voce_launch_begin_increment(&d, c->stream);
+    int e = voce_launch_begin_increment(&c->L, c->stream);
     if (e) return VOCE_ERR_DEVICE;
     const size_t ctl_off = offsetof(MacroState, istep);
     // enqueue iterations in chunks; kernels past convergence return at once
@@ -740,7 +769,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             for (size_t i = old; i < c->ev.size(); ++i) CK(hipEventCreate(&c->ev[i]));
         }
         for (int i = 0; i < n; ++i) {
-            e = voce_launch_iteration(&d, c->stream, c->prof ? (void *)&c->ev[6 * i] : nullptr);
+            e = voce_launch_iteration(&c->L, c->stream, c->prof ? (void *)&c->ev[6 * i] : nullptr);
             if (e) return VOCE_ERR_DEVICE;
         }
         launched += n;
@@ -765,7 +794,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
     }
     const int istep = c->h_ctl[0];
     const int need_skew = (c->update_flag == 1 && istep > 1) || c->creep || c->diag;
-    e = voce_launch_closeout(&d, c->stream, need_skew);
+    e = voce_launch_closeout(&c->L, c->stream, need_skew);
     if (e) return VOCE_ERR_DEVICE;
     CK(hipMemcpyAsync(c->h_ms, d.ms, sizeof(MacroState), hipMemcpyDeviceToHost, c->stream));
     CK(hipStreamSynchronize(c->stream));

@@ -31,7 +31,9 @@
 //                             skew part of the Green update, components
 //                             (2,1), (0,2), (1,0) (lattice-spin recovery)
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
+#include <hip/hip_runtime.h>   // int2
 
 #define VOCE_MAXIT_LOG 4096
 #define VOCE_GAS 0xffffffffu
@@ -95,18 +97,35 @@ struct RveDev {
     int fft_f32;                 // 1: FFT in FP32
     int uniform_phase;           // 1: one crystalline phase (its constants are uniform)
     int two_crss;                // 1: some phase keeps a separate backward CRSS
+    int tk;                      // (y,kz) columns per block of the x-pass kernel
+};
+
+// What one launch sequence covers: a device table of RVE descriptors and,
+// for a batch, per-kernel maps block -> (RVE, local block).  A single RVE
+// passes a one-entry table and null maps.  Batch-wide template choices:
+// fft_f32 must agree across the batch; uniform_phase is the AND and
+// two_crss the OR of the members'.
+struct LaunchSet {
+    const RveDev *tab;
+    const int2 *map_plane6, *map_plane3, *map_x, *map_vox, *map_one;
+    int n_plane6, n_plane3, n_x, n_vox, n_one;
+    int large;
+    size_t pl_lds, x_lds;
+    int fft_f32, uniform_phase, two_crss;
 };
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 // host-side launchers (voce_kernels.hip); each returns a hipError_t as int
-int voce_launch_begin_increment(const RveDev *r, void *stream);
-int voce_launch_iteration(const RveDev *r, void *stream, void *events /* hipEvent_t[6] or NULL */);
-int voce_launch_closeout(const RveDev *r, void *stream, int need_skew);
-int voce_launch_init_fields(const RveDev *r, const double *sig6_by_orient, void *stream);
+int voce_launch_begin_increment(const LaunchSet *L, void *stream);
+int voce_launch_iteration(const LaunchSet *L, void *stream, void *events /* hipEvent_t[6] or NULL */);
+int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew);
+int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 int voce_kernels_prepare(void);
+int voce_xgreen_tile(int nx);
+void voce_launch_shape(const RveDev *r, int *large, size_t *pl_lds, size_t *x_lds);
 #ifdef __cplusplus
 }
 #endif

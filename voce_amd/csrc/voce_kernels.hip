@@ -203,6 +203,19 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
     for (int i = 0; i < n2; ++i) { fft_stage<C, 2, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 2; }
 }
 
+// Kernels take a table of RVE descriptors and an optional block map.  A
+// single-RVE launch passes a one-entry table and no map (local block =
+// blockIdx.x); a batched launch maps every block to (RVE, local block), so
+// one grid covers RVEs of different sizes.
+#define RVE_PROLOGUE(LB)                                                     \
+    int rid_ = 0, LB = (int)blockIdx.x;                                      \
+    if (bmap) {                                                              \
+        const int2 m_ = bmap[blockIdx.x];                                    \
+        rid_ = m_.x;                                                         \
+        LB = m_.y;                                                           \
+    }                                                                        \
+    const RveDev &r = rtab[rid_]
+
 __device__ __forceinline__ bool iter_active(const MacroState *ms)
 {
     return ms->active != 0;
@@ -213,16 +226,17 @@ __device__ __forceinline__ bool iter_active(const MacroState *ms)
 // grid (nx, 6), dynamic LDS (ny*nzh + ny + nz) complex
 // ------------------------------------------------------------------
 template <typename T, int NT, int MAXEL>
-__global__ __launch_bounds__(NT) void k_plane_fwd(RveDev r)
+__global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
     if (!iter_active(r.ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
     C *twy = s + ny * nzh;
     C *twz = twy + ny;
-    const int x = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const C *gty = (const C *)r.tw_y;
     const C *gtz = (const C *)r.tw_z;
     for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
@@ -275,16 +289,17 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(RveDev r)
 // mode 1: WF[c] = result (skew recovery).  grid (nx, ncomp)
 // ------------------------------------------------------------------
 template <typename T, int NT, int MAXEL>
-__global__ __launch_bounds__(NT) void k_plane_inv(RveDev r, int mode)
+__global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
 {
     using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
     if (mode == 0 && !iter_active(r.ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
     C *twy = s + ny * nzh;
     C *twz = twy + ny;
-    const int x = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const C *gty = (const C *)r.tw_y;
     const C *gtz = (const C *)r.tw_z;
     for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
@@ -442,9 +457,11 @@ __device__ __forceinline__ void green_apply(const double *C0, const T n[3], cons
 // mode 1 (skew): USUM (3 comps, already in k space) -> inverse x FFT -> SPEC[0..2]
 // ------------------------------------------------------------------
 template <typename T, int NT, int MAXEL>
-__global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
+__global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
 {
     using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
+    const int tk = r.tk;
     const MacroState *ms = r.ms;
     if (mode == 0 && !iter_active(ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -455,7 +472,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(RveDev r, int mode, int tk)
     const int tid = threadIdx.x;
     const C *gtx = (const C *)r.tw_x;
     for (int i = tid; i < nx; i += NT) twx[i] = gtx[i];
-    const int col0 = blockIdx.x * tk;
+    const int col0 = lb * tk;
     const int ncol = min(tk, ncolt - col0);
     const int ncomp = (mode == 0) ? 6 : 3;
     const C *src = (const C *)((mode == 0) ? r.spec : r.usum);
@@ -1158,8 +1175,9 @@ __device__ __forceinline__ void rotate_sym_b5(const double R[9], const double *p
 }
 
 template <int NT, bool UNI, bool TWO>
-__global__ __launch_bounds__(NT) void k_nr(RveDev r)
+__global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
+    RVE_PROLOGUE(lb);
     MacroState *ms = r.ms;
     if (!iter_active(ms)) return;
     // per-lane values that stay constant through the Newton loop live in LDS
@@ -1169,7 +1187,7 @@ __global__ __launch_bounds__(NT) void k_nr(RveDev r)
     __shared__ double lds_R[9][NT];     // TranMat of the voxel
     __shared__ double lds_cr[TWO ? 24 : 12][NT];   // forward (and backward) CRSS
     const long V = r.V;
-    const long v = (long)blockIdx.x * NT + threadIdx.x;
+    const long v = (long)lb * NT + threadIdx.x;
     double acc[14];
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
@@ -1387,16 +1405,16 @@ __global__ __launch_bounds__(NT) void k_nr(RveDev r)
             }
         }
     }
-    block_reduce_store<NT, 14>(acc, r.part + (long)blockIdx.x * 16);
+    block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
     // last block finalises the iteration
     __shared__ int amlast;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) amlast = (atomicAdd(&ms->ticket, 1) == (int)gridDim.x - 1);
+    if (threadIdx.x == 0) amlast = (atomicAdd(&ms->ticket, 1) == r.nblk_vox - 1);
     __syncthreads();
     if (!amlast) return;
     __shared__ double tot[16];
-    sum_partials<NT, 14>(r.part, gridDim.x, tot);
+    sum_partials<NT, 14>(r.part, r.nblk_vox, tot);
     if (threadIdx.x == 0) {
         ms->ticket = 0;
         finalize_iteration(ms, tot);
@@ -1404,8 +1422,10 @@ __global__ __launch_bounds__(NT) void k_nr(RveDev r)
 }
 
 // increment start (evolution.c:306-340)
-__global__ void k_begin_increment(RveDev r)
+__global__ void k_begin_increment(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
+    RVE_PROLOGUE(lb);
+    (void)lb;
     MacroState *ms = r.ms;
     ms->istep += 1;
     ms->iter = 0;
@@ -1468,11 +1488,12 @@ __device__ void orientation(double tran[9], const double rot_in[9])
 // block: [0..4] sum Eps (B basis), [5..9] sum Edot.
 // ------------------------------------------------------------------
 template <int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
+__global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int rot_on)
 {
+    RVE_PROLOGUE(lb);
     const MacroState *ms = r.ms;
     const long V = r.V;
-    const long v = (long)blockIdx.x * NT + threadIdx.x;
+    const long v = (long)lb * NT + threadIdx.x;
     double acc[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) acc[i] = 0.0;
@@ -1683,13 +1704,15 @@ __global__ __launch_bounds__(NT) void k_closeout(RveDev r, int rot_on)
             }
         }
     }
-    block_reduce_store<NT, 10>(acc, r.part + (long)blockIdx.x * 16);
+    block_reduce_store<NT, 10>(acc, r.part + (long)lb * 16);
 }
 
 // increment finaliser (evolution.c:438-498)
 template <int NT>
-__global__ __launch_bounds__(NT) void k_fin_incr(RveDev r)
+__global__ __launch_bounds__(NT) void k_fin_incr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
+    RVE_PROLOGUE(lb);
+    (void)lb;
     MacroState *ms = r.ms;
     __shared__ double tot[16];
     sum_partials<NT, 10>(r.part, r.nblk_vox, tot);
@@ -1722,8 +1745,9 @@ __global__ __launch_bounds__(NT) void k_fin_incr(RveDev r)
 }
 
 // initial fields: S6 from the per-orientation table (InitFields, init.c:486-534)
-__global__ void k_init_sig(RveDev r, const double *sig6_by_orient)
+__global__ void k_init_sig(const RveDev *__restrict__ rtab, const double *sig6_by_orient)
 {
+    const RveDev r = rtab[0];
     const long V = r.V;
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -1738,49 +1762,34 @@ constexpr int NT_PL_L = 512, MAXEL_PL_L = 8320;   // planes up to 128 x 128
 constexpr int NT_X = 256, MAXEL_X = 3072;
 
 template <typename T>
-int xgreen_tk(const RveDev *r)
+hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
 {
-    int tk = MAXEL_X / (6 * r->nx);
-    if (tk > 16) tk = 16;
-    if (tk < 1) tk = 1;
-    return tk;
-}
-
-template <typename T>
-hipError_t launch_iter_t(const RveDev *r, hipStream_t st, hipEvent_t *ev)
-{
-    using C = typename cplx_of<T>::t;
     int evi = 0;
 #define MARK()                                                   \
     do {                                                         \
         if (ev) (void)hipEventRecord(ev[evi++], st);             \
     } while (0)
     MARK();
-    const size_t pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * sizeof(C);
-    const bool large = (r->ny * r->nzh > MAXEL_PL_S);
-    if (large) {
-        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 6), dim3(NT_PL_L), pl_lds, st, *r);
+    if (L->large) {
+        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6);
     } else {
-        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 6), dim3(NT_PL_S), pl_lds, st, *r);
+        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6);
     }
     MARK();
-    const int tk = xgreen_tk<T>(r);
-    const int nblk = (r->ny * r->nzh + tk - 1) / tk;
-    const size_t x_lds = (size_t)(6 * tk * r->nx + r->nx) * sizeof(C);
-    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(nblk), dim3(NT_X), x_lds, st, *r, 0, tk);
+    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, 0);
     MARK();
-    if (large) {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 6), dim3(NT_PL_L), pl_lds, st, *r, 0);
+    if (L->large) {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6, 0);
     } else {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 6), dim3(NT_PL_S), pl_lds, st, *r, 0);
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6, 0);
     }
     MARK();
-    if (r->uniform_phase) {
-        if (r->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+    if (L->uniform_phase) {
+        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     } else {
-        if (r->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r);
+        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     }
     MARK();
     MARK();
@@ -1789,40 +1798,38 @@ hipError_t launch_iter_t(const RveDev *r, hipStream_t st, hipEvent_t *ev)
 }
 
 template <typename T>
-hipError_t launch_skew_t(const RveDev *r, hipStream_t st)
+hipError_t launch_skew_t(const LaunchSet *L, hipStream_t st)
 {
-    using C = typename cplx_of<T>::t;
-    const int tk = xgreen_tk<T>(r);
-    const int nblk = (r->ny * r->nzh + tk - 1) / tk;
-    const size_t x_lds = (size_t)(6 * tk * r->nx + r->nx) * sizeof(C);
-    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(nblk), dim3(NT_X), x_lds, st, *r, 1, tk);
-    const size_t pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * sizeof(C);
-    if (r->ny * r->nzh > MAXEL_PL_S) {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(r->nx, 3), dim3(NT_PL_L), pl_lds, st, *r, 1);
+    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, 1);
+    if (L->large) {
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane3), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane3, 1);
     } else {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(r->nx, 3), dim3(NT_PL_S), pl_lds, st, *r, 1);
+        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane3), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane3, 1);
     }
     return hipGetLastError();
 }
 
 template <typename T>
-void set_lds_limits()
+hipError_t set_lds_limits()
 {
     const int big = 160 * 1024;
-    hipFuncSetAttribute((const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    hipFuncSetAttribute((const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    hipFuncSetAttribute((const void *)k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    hipFuncSetAttribute((const void *)k_plane_inv<T, NT_PL_S, MAXEL_PL_S>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    hipFuncSetAttribute((const void *)k_xgreen<T, NT_X, MAXEL_X>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    const void *fns[] = {(const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, (const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>,
+                         (const void *)k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>, (const void *)k_plane_inv<T, NT_PL_S, MAXEL_PL_S>,
+                         (const void *)k_xgreen<T, NT_X, MAXEL_X>};
+    for (const void *f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
 
 extern "C" int voce_kernels_prepare(void)
 {
-    set_lds_limits<double>();
-    set_lds_limits<float>();
-    return 0;
+    hipError_t e = set_lds_limits<double>();
+    if (e == hipSuccess) e = set_lds_limits<float>();
+    return (int)e;
 }
 
 // 2^a 3^b, 2..128 (y and z must also be even: the plane kernels pack row pairs)
@@ -1835,34 +1842,51 @@ extern "C" int voce_grid_supported(int n)
     return m == 1;
 }
 
-extern "C" int voce_launch_begin_increment(const RveDev *r, void *stream)
+extern "C" int voce_xgreen_tile(int nx)
 {
-    hipLaunchKernelGGL(k_begin_increment, dim3(1), dim3(1), 0, (hipStream_t)stream, *r);
+    int tk = MAXEL_X / (6 * nx);
+    if (tk > 16) tk = 16;
+    if (tk < 1) tk = 1;
+    return tk;
+}
+
+// Launch classes of one RVE (plane-kernel size class and dynamic LDS).
+extern "C" void voce_launch_shape(const RveDev *r, int *large, size_t *pl_lds, size_t *x_lds)
+{
+    const size_t cb = r->fft_f32 ? 8 : 16;
+    *large = (r->ny * r->nzh > MAXEL_PL_S);
+    *pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * cb;
+    *x_lds = (size_t)(6 * r->tk * r->nx + r->nx) * cb;
+}
+
+extern "C" int voce_launch_begin_increment(const LaunchSet *L, void *stream)
+{
+    hipLaunchKernelGGL(k_begin_increment, dim3(L->n_one), dim3(1), 0, (hipStream_t)stream, L->tab, L->map_one);
     return (int)hipGetLastError();
 }
 
-extern "C" int voce_launch_iteration(const RveDev *r, void *stream, void *events)
+extern "C" int voce_launch_iteration(const LaunchSet *L, void *stream, void *events)
 {
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t *ev = (hipEvent_t *)events;
-    return (int)(r->fft_f32 ? launch_iter_t<float>(r, st, ev) : launch_iter_t<double>(r, st, ev));
+    return (int)(L->fft_f32 ? launch_iter_t<float>(L, st, ev) : launch_iter_t<double>(L, st, ev));
 }
 
-extern "C" int voce_launch_closeout(const RveDev *r, void *stream, int need_skew)
+extern "C" int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew)
 {
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSuccess;
-    if (need_skew) e = r->fft_f32 ? launch_skew_t<float>(r, st) : launch_skew_t<double>(r, st);
+    if (need_skew) e = L->fft_f32 ? launch_skew_t<float>(L, st) : launch_skew_t<double>(L, st);
     if (e != hipSuccess) return (int)e;
-    if (r->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
-    else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(r->nblk_vox), dim3(NT_VOX), 0, st, *r, need_skew);
-    hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(1), dim3(NT_FIN), 0, st, *r);
+    if (L->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox, need_skew);
+    else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox, need_skew);
+    hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, st, L->tab, L->map_one);
     return (int)hipGetLastError();
 }
 
-extern "C" int voce_launch_init_fields(const RveDev *r, const double *sig6_by_orient, void *stream)
+extern "C" int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream)
 {
-    const long nb = (r->V + 255) / 256;
-    hipLaunchKernelGGL(k_init_sig, dim3(nb), dim3(256), 0, (hipStream_t)stream, *r, sig6_by_orient);
+    const long nb = (V + 255) / 256;
+    hipLaunchKernelGGL(k_init_sig, dim3(nb), dim3(256), 0, (hipStream_t)stream, tab1, sig6_by_orient);
     return (int)hipGetLastError();
 }
