@@ -17,6 +17,7 @@
 #include "../../include/voce.h"
 #include "../../include/voce_prof.h"
 #include "voce_internal.h"
+#include "voce_host.h"
 
 namespace {
 
@@ -233,32 +234,7 @@ void voigt_to_full(const double v[6], double m[3][3])
 
 }  // namespace
 
-struct voce_ctx {
-    int dims[3];
-    long V;
-    int precision;
-    int n_steps_done;
-    int diag;
-    int pred_iters;
-    int update_flag, creep, iter_max;
-    hipStream_t stream;
-    RveDev d;              // host copy of the descriptor
-    RveDev *dtab;          // device copy (a one-entry RVE table)
-    LaunchSet L;           // launch sequence over that table
-    MacroState *h_ms;      // pinned mirror of the device MacroState
-    int *h_ctl;            // pinned: istep, iter, iter_total, active, error
-    int two_crss_any;
-    std::vector<int> grain, phase_id;
-    std::vector<double> log_e, log_s, log_v;
-    std::vector<void *> allocs;
-    // kernel timing (include/voce_prof.h)
-    int prof;
-    std::vector<hipEvent_t> ev;    // 6 per enqueued iteration
-    double prof_ms[VOCE_PROF_NK];
-    long prof_n[VOCE_PROF_NK];
-};
-
-static int dev_alloc(voce_ctx *c, void **p, size_t bytes)
+int voce_dev_alloc(voce_ctx *c, void **p, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     hipError_t e = hipMalloc(p, bytes);
@@ -268,9 +244,10 @@ static int dev_alloc(voce_ctx *c, void **p, size_t bytes)
 }
 
 // Push the host descriptor to its device copy and refresh the launch set.
-static int sync_tab(voce_ctx *c)
+int voce_sync_tab(voce_ctx *c)
 {
     const RveDev &d = c->d;
+    if (c->on_tab) *c->on_tab = d;
     if (hipMemcpyAsync(c->dtab, &d, sizeof(RveDev), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return VOCE_ERR_DEVICE;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return VOCE_ERR_DEVICE;
@@ -282,7 +259,11 @@ static int sync_tab(voce_ctx *c)
     L.n_x = (d.ny * d.nzh + d.tk - 1) / d.tk;
     L.n_vox = d.nblk_vox;
     L.n_one = 1;
-    voce_launch_shape(&d, &L.large, &L.pl_lds, &L.x_lds);
+    L.plv = d.plv;
+    L.xv = d.xv;
+    L.pl_lds = voce_plane_lds(&d, d.plv);
+    L.x_lds = voce_xgreen_lds(&d, d.xv, d.tk);
+    if (!L.pl_lds || !L.x_lds) return VOCE_ERR_UNSUPPORTED_GRID;
     L.fft_f32 = d.fft_f32;
     L.uniform_phase = d.uniform_phase;
     L.two_crss = d.two_crss;
@@ -332,7 +313,7 @@ extern "C" void voce_ctx_destroy(voce_ctx *c)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->h_ms) (void)hipHostFree(c->h_ms);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -395,6 +376,11 @@ static int setup_phase(const voce_phase &in, PhaseConst &pc, double C4[3][3][3][
 
 extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
 {
+    return voce_ctx_build(su, nullptr, out);
+}
+
+int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **out)
+{
     if (!out) return VOCE_ERR_ARG;
     *out = nullptr;
     if (!su || !su->euler_deg || !su->grain || !su->phase_id) return VOCE_ERR_ARG;
@@ -402,7 +388,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     if (nx <= 0 || ny <= 0 || nz <= 0 || su->n_phases < 1 || su->n_phases > 2) return VOCE_ERR_ARG;
     if (!voce_grid_supported(nx) || !voce_grid_supported(ny) || !voce_grid_supported(nz)) return VOCE_ERR_UNSUPPORTED_GRID;
     if (ny % 2 || nz % 2) return VOCE_ERR_UNSUPPORTED_GRID;   // the plane kernels pack row pairs
-    if (su->iter_max > VOCE_MAXIT_LOG) return VOCE_ERR_ARG;
+    if (su->iter_max < 0) return VOCE_ERR_ARG;
     const long V = (long)nx * ny * nz;
     const double WGT = 1. / V;
 
@@ -478,27 +464,14 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     c66_to_c4(C066, C0);
     c66_to_c4(S066, S0);
     // per-orientation table (stride VOCE_OTAB): S_local = C_gr^-1, then
-    // A^-1 and A^-1 C066 with A = I + C066 S_local, the constant part of the
-    // Newton Jacobian of UpdateStress (evolution.c:85-93)
+    // K_c = C066^-1 + S_local, the constant part of the compliance-form
+    // Newton matrix of UpdateStress (evolution.c:85-93 with C066 factored out)
     std::vector<double> otab(VOCE_OTAB * (norient > 0 ? norient : 1), 0.0);
-    for (long o = 0; o < norient; ++o) {
-        double A[6][6];
-        for (int i = 0; i < 36; ++i) otab[VOCE_OTAB * o + i] = Sloc[36 * o + i];
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) {
-                double t = (double)(i == j);
-                for (int k = 0; k < 6; ++k) t += C066[i][k] * Sloc[36 * o + k * 6 + j];
-                A[i][j] = t;
-            }
-        if (lu_inv66(A)) { free(ms0); return VOCE_ERR_SINGULAR; }
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) {
-                otab[VOCE_OTAB * o + 36 + i * 6 + j] = A[i][j];
-                double t = 0.0;
-                for (int k = 0; k < 6; ++k) t += A[i][k] * C066[k][j];
-                otab[VOCE_OTAB * o + 72 + i * 6 + j] = t;
-            }
-    }
+    for (long o = 0; o < norient; ++o)
+        for (int i = 0; i < 36; ++i) {
+            otab[VOCE_OTAB * o + i] = Sloc[36 * o + i];
+            otab[VOCE_OTAB * o + 36 + i] = S066[i / 6][i % 6] + Sloc[36 * o + i];
+        }
 
     // boundary conditions (init.c:713-745)
     const int *vf = su->velgrad_flag, *sf = su->stress_flag;
@@ -557,7 +530,10 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     M.wgt = WGT;
     const double Wgt_ph1 = (1.0 * (double)nph1) / V;
     M.wgt_over_ph1 = WGT / Wgt_ph1;
-    for (int i = 0; i < 36; ++i) M.C066[i] = C066[i / 6][i % 6];
+    for (int i = 0; i < 36; ++i) {
+        M.C066[i] = C066[i / 6][i % 6];
+        M.S066[i] = S066[i / 6][i % 6];
+    }
     memcpy(M.S0, S0, sizeof(M.S0));
     memcpy(M.C0, C0, sizeof(M.C0));
     for (int i = 0; i < 9; ++i) {
@@ -598,11 +574,18 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
 #define ALLOC(ptr, bytes)                                             \
     do {                                                              \
         void *p_ = nullptr;                                           \
-        rc = dev_alloc(c, &p_, (bytes));                              \
+        rc = voce_dev_alloc(c, &p_, (bytes));                              \
         if (rc) { free(ms0); voce_ctx_destroy(c); return rc; }        \
         ptr = (decltype(ptr))p_;                                      \
     } while (0)
-    if (hipStreamCreate(&c->stream) != hipSuccess) { free(ms0); voce_ctx_destroy(c); return VOCE_ERR_DEVICE; }
+    c->log_cap = su->iter_max > 0 ? su->iter_max : 1;
+    if (opt && opt->stream) {
+        c->stream = opt->stream;
+        c->own_stream = false;
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess) { free(ms0); voce_ctx_destroy(c); return VOCE_ERR_DEVICE; }
+        c->own_stream = true;
+    }
     if (voce_kernels_prepare() != 0) { free(ms0); voce_ctx_destroy(c); return VOCE_ERR_DEVICE; }
     RveDev &d = c->d;
     memset(&d, 0, sizeof(d));
@@ -613,7 +596,11 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     // one crystalline phase, numbered 1: its constants are uniform across a wave
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;
-    d.tk = voce_xgreen_tile(nx);
+    // default FFT launch choices (a tuned set replaces them)
+    d.plv = (ny * d.nzh <= 2112) ? 1 : 2;
+    d.xv = 0;
+    d.tk = voce_xgreen_tile(nx, 0);
+    d.pl_r8 = d.x_r8 = 1;
     const size_t cb = d.fft_f32 ? 8 : 16;
     const size_t db = sizeof(double);
     ALLOC(d.E6, 6 * V * db);
@@ -633,15 +620,26 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
     ALLOC(d.PH, V);
     ALLOC(d.SLOC, otab.size() * db);
     ALLOC(d.phase, 2 * sizeof(PhaseConst));
-    ALLOC(d.ms, sizeof(MacroState));
-    ALLOC(d.spec, 6 * d.nspec * cb);
-    ALLOC(d.usum, 3 * d.nspec * cb);
-    d.nblk_vox = (int)((V + 255) / 256);
-    ALLOC(d.part, (size_t)d.nblk_vox * 16 * db);
+    if (opt && opt->ms_dev) {
+        d.ms = opt->ms_dev;
+        c->logs_dev = opt->logs_dev;
+    } else {
+        ALLOC(d.ms, sizeof(MacroState));
+        ALLOC(c->logs_dev, 3 * (size_t)c->log_cap * db);
+    }
+    ms0->logs = c->logs_dev;
+    ms0->log_cap = c->log_cap;
+    d.nblk_vox = (int)((V + voce_vox_block() - 1) / voce_vox_block());
+    if (!(opt && opt->no_workspace)) {
+        ALLOC(d.spec, 6 * d.nspec * cb);
+        ALLOC(d.usum, 3 * d.nspec * cb);
+        ALLOC(d.part, (size_t)d.nblk_vox * 16 * db);
+    }
     ALLOC(d.tw_x, nx * cb);
     ALLOC(d.tw_y, ny * cb);
     ALLOC(d.tw_z, nz * cb);
     ALLOC(c->dtab, sizeof(RveDev));
+    if (opt) c->on_tab = opt->on_tab;
     double *sig6o_d = nullptr;
     ALLOC(sig6o_d, sig6o.size() * db);
     if (hipHostMalloc((void **)&c->h_ms, sizeof(MacroState), hipHostMallocDefault) != hipSuccess ||
@@ -683,7 +681,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
         ZERO(d.WF, 3 * V * db);
         ZERO(d.GAC, V * db);
         ZERO(d.GTOT, 12 * V * db);
-        ZERO(d.usum, 3 * d.nspec * cb);
+        if (d.usum) ZERO(d.usum, 3 * d.nspec * cb);
         if (d.fft_f32) {
             std::vector<float> tx, ty, tz;
             make_twiddles(nx, tx); make_twiddles(ny, ty); make_twiddles(nz, tz);
@@ -695,7 +693,7 @@ extern "C" int voce_ctx_create(const voce_setup *su, voce_ctx **out)
             H2D(d.tw_x, tx.data(), nx * cb); H2D(d.tw_y, ty.data(), ny * cb); H2D(d.tw_z, tz.data(), nz * cb);
             CK(hipStreamSynchronize(c->stream));
         }
-        e = sync_tab(c);
+        e = voce_sync_tab(c);
         if (e) break;
         e = voce_launch_init_fields(c->dtab, V, sig6o_d, c->stream);
         if (e) break;
@@ -720,11 +718,11 @@ static int ensure_diag(voce_ctx *c)
                                                 {&d.EVM, V * db}, {&d.WORKP, V * db}, {&d.RHOR, V * db},
                                                 {&d.RHOC, V * db}};
     for (auto &w : want) {
-        if (dev_alloc(c, &p, w.n)) return VOCE_ERR_NOMEM;
+        if (voce_dev_alloc(c, &p, w.n)) return VOCE_ERR_NOMEM;
         if (hipMemsetAsync(p, 0, w.n, c->stream) != hipSuccess) return VOCE_ERR_DEVICE;
         *w.dst = (double *)p;
     }
-    return sync_tab(c);
+    return voce_sync_tab(c);
 }
 
 extern "C" int voce_set_diagnostics(voce_ctx *c, int on)
@@ -780,7 +778,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             // only the iterations that ran (the rest returned at the flag check)
             const int ran = c->h_ctl[1] - done_before;
             for (int i = 0; i < ran && i < n; ++i)
-                for (int k = 0; k < VOCE_PROF_NK; ++k) {
+                for (int k = 0; k < VOCE_PROF_NK; ++k) {   // events ev[0..5] bracket the 5 kernels
                     float ms = 0.f;
                     if (hipEventElapsedTime(&ms, c->ev[6 * i + k], c->ev[6 * i + k + 1]) == hipSuccess) {
                         c->prof_ms[k] += ms;
@@ -803,27 +801,45 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
     c->pred_iters = M.iter > 0 ? M.iter : 1;
     c->n_steps_done = M.istep;
     if (rep) {
-        memset(rep, 0, sizeof(*rep));
-        rep->istep = M.istep;
-        rep->iter = M.iter;
-        rep->iter_total = M.iter_total;
-        rep->e_vm = M.e_vm; rep->evmp = M.evmp; rep->d_vm = M.d_vm; rep->dvmp = M.dvmp;
-        rep->s_vm = M.s_vm; rep->s_vm1 = M.s_vm1;
-        memcpy(rep->sig_avg, M.SigAvg, 72);
-        memcpy(rep->sig_avg1, M.SigAvg1, 72);
-        memcpy(rep->disgrad_avg, M.DisGradAvg_actual, 72);
-        memcpy(rep->velgrad_avg, M.VelGradAvg, 72);
-        memcpy(rep->eps_avg, M.EpsAvg, 72);
-        memcpy(rep->edot_avg, M.EdotAvg, 72);
-        const int n = M.iter < VOCE_MAXIT_LOG ? M.iter : VOCE_MAXIT_LOG;
-        c->log_e.assign(M.log_err_e, M.log_err_e + n);
-        c->log_s.assign(M.log_err_s, M.log_err_s + n);
-        c->log_v.assign(M.log_svm, M.log_svm + n);
-        rep->err_e = c->log_e.data();
-        rep->err_s = c->log_s.data();
-        rep->svm_it = c->log_v.data();
+        const int n = M.iter < c->log_cap ? M.iter : c->log_cap;
+        int lrc = voce_read_logs(c, n);
+        if (lrc) return lrc;
+        voce_fill_report(c, M, rep);
     }
     return VOCE_OK;
+}
+
+int voce_read_logs(voce_ctx *c, int n)
+{
+    c->log_e.resize(n);
+    c->log_s.resize(n);
+    c->log_v.resize(n);
+    if (n <= 0) return 0;
+    const size_t b = (size_t)n * sizeof(double);
+    CK(hipMemcpyAsync(c->log_e.data(), c->logs_dev, b, hipMemcpyDeviceToHost, c->stream));
+    CK(hipMemcpyAsync(c->log_s.data(), c->logs_dev + c->log_cap, b, hipMemcpyDeviceToHost, c->stream));
+    CK(hipMemcpyAsync(c->log_v.data(), c->logs_dev + 2 * c->log_cap, b, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void voce_fill_report(voce_ctx *c, const MacroState &M, voce_incr_report *rep)
+{
+    memset(rep, 0, sizeof(*rep));
+    rep->istep = M.istep;
+    rep->iter = M.iter;
+    rep->iter_total = M.iter_total;
+    rep->e_vm = M.e_vm; rep->evmp = M.evmp; rep->d_vm = M.d_vm; rep->dvmp = M.dvmp;
+    rep->s_vm = M.s_vm; rep->s_vm1 = M.s_vm1;
+    memcpy(rep->sig_avg, M.SigAvg, 72);
+    memcpy(rep->sig_avg1, M.SigAvg1, 72);
+    memcpy(rep->disgrad_avg, M.DisGradAvg_actual, 72);
+    memcpy(rep->velgrad_avg, M.VelGradAvg, 72);
+    memcpy(rep->eps_avg, M.EpsAvg, 72);
+    memcpy(rep->edot_avg, M.EdotAvg, 72);
+    rep->err_e = c->log_e.data();
+    rep->err_s = c->log_s.data();
+    rep->svm_it = c->log_v.data();
 }
 
 static void b6_to_tensor6(const double *a, double out[6])

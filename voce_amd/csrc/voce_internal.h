@@ -35,9 +35,8 @@
 #include <stdint.h>
 #include <hip/hip_runtime.h>   // int2
 
-#define VOCE_MAXIT_LOG 4096
 #define VOCE_GAS 0xffffffffu
-#define VOCE_OTAB 108   // doubles per orientation-table row
+#define VOCE_OTAB 72    // doubles per orientation-table row
 
 struct PhaseConst {
     int nsys;
@@ -57,6 +56,7 @@ struct MacroState {
     // constants
     double dt, err_tol, wgt, wgt_over_ph1;   // wgt_over_ph1 = WGT/Wgt_ph1
     double C066[36];
+    double S066[36];        // C066^-1 (B basis)
     double S0[81];
     double C0[81];
     double Udot[9], Udot_a[9], Scauchy[9];
@@ -74,8 +74,12 @@ struct MacroState {
     double creep_sym6[6];    // creep mode: uniform sym(DisGrad) at predictor
     double creep_skew[3];    // creep mode: uniform skew(DisGrad) at predictor
     int istep, iter, iter_total, active;
-    int error, ticket, pad2, pad3;   // ticket: blocks of k_nr done
-    double log_err_e[VOCE_MAXIT_LOG], log_err_s[VOCE_MAXIT_LOG], log_svm[VOCE_MAXIT_LOG];
+    int error, pad_t, log_cap, vg_pending;
+    // per-iteration Err_e, Err_s, s_vm of the running increment: 3 x log_cap
+    double *logs;
+    // velocity gradient for the next increment (taken by k_begin_increment
+    // when vg_pending is set; a request-driven caller sets it per increment)
+    double vg_next[9];
 };
 
 // Device pointers of one RVE.
@@ -86,8 +90,8 @@ struct RveDev {
     double *GAMDOT, *TAUX, *SVM, *EVM, *WORKP, *RHOR, *RHOC;   // diagnostics (may be null)
     uint32_t *ORI;
     uint8_t *PH;
-    double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1,
-                                 // A^-1 and A^-1 C066 (36 each), A = I + C066 S_local
+    double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1
+                                 // and K_c = C066^-1 + S_local (36 each, row major)
     PhaseConst *phase;           // [2]
     MacroState *ms;
     void *spec, *usum;           // complex T
@@ -97,7 +101,11 @@ struct RveDev {
     int fft_f32;                 // 1: FFT in FP32
     int uniform_phase;           // 1: one crystalline phase (its constants are uniform)
     int two_crss;                // 1: some phase keeps a separate backward CRSS
-    int tk;                      // (y,kz) columns per block of the x-pass kernel
+    // FFT launch choices (tuned per grid size): plane-kernel variant plv and
+    // x-pass variant xv (0..2, see voce_kernels.hip), x-pass tile tk
+    // ((y,kz) columns per block), radix-8 stages allowed in the plane and
+    // x passes
+    int plv, xv, tk, pl_r8, x_r8;
 };
 
 // What one launch sequence covers: a device table of RVE descriptors and,
@@ -109,8 +117,8 @@ struct LaunchSet {
     const RveDev *tab;
     const int2 *map_plane6, *map_plane3, *map_x, *map_vox, *map_one;
     int n_plane6, n_plane3, n_x, n_vox, n_one;
-    int large;
-    size_t pl_lds, x_lds;
+    int plv, xv;                 // kernel variants (same for every RVE of the set)
+    size_t pl_lds, x_lds;        // dynamic LDS: the largest over the set's RVEs
     int fft_f32, uniform_phase, two_crss;
 };
 
@@ -124,8 +132,10 @@ int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 int voce_kernels_prepare(void);
-int voce_xgreen_tile(int nx);
-void voce_launch_shape(const RveDev *r, int *large, size_t *pl_lds, size_t *x_lds);
+int voce_xgreen_tile(int nx, int xv);
+int voce_vox_block(void);   // threads per block of the per-voxel kernels
+size_t voce_plane_lds(const RveDev *r, int plv);            // 0: does not fit
+size_t voce_xgreen_lds(const RveDev *r, int xv, int tk);    // 0: does not fit
 #ifdef __cplusplus
 }
 #endif

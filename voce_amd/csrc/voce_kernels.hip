@@ -180,9 +180,10 @@ __device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int l
     __syncthreads();
 }
 
-// Mixed-radix plan: radix-3 stages, then radix 8/4/2 for the power of two.
+// Mixed-radix plan: radix-3 stages, then radix 8/4/2 for the power of two
+// (r8 = 0: radix 4/2 only; the choice is tuned per grid size).
 template <typename C, int NT, int MAXEL, bool INV, bool LF>
-__device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, const C *tw)
+__device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, const C *tw, int r8)
 {
     int Ns = 1, rem = N;
     while (rem % 3 == 0) {
@@ -197,6 +198,11 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
     if (a % 3 == 1) {
         if (n8 >= 1) { n8 -= 1; n4 = 2; }
         else n2 = 1;
+    }
+    if (!r8) {
+        n8 = 0;
+        n4 = a / 2;
+        n2 = a % 2;
     }
     for (int i = 0; i < n8; ++i) { fft_stage<C, 8, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 8; }
     for (int i = 0; i < n4; ++i) { fft_stage<C, 4, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw); Ns *= 4; }
@@ -219,6 +225,13 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
 __device__ __forceinline__ bool iter_active(const MacroState *ms)
 {
     return ms->active != 0;
+}
+
+// The increment's spin is needed by update_orient from step 2
+// (evolution.c:500), by the creep predictor and by the diagnostics.
+__device__ __forceinline__ bool rve_need_skew(const MacroState *ms)
+{
+    return (ms->update_flag == 1 && ms->istep > 1) || ms->creep || ms->diag;
 }
 
 // ------------------------------------------------------------------
@@ -248,7 +261,7 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)] = (T)src[e];
     }
     __syncthreads();
-    fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, 2 * nzh, 1, twz);
+    fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, 2 * nzh, 1, twz, r.pl_r8);
     // split packed rows a + i b into the half spectra of a and b
     {
         constexpr int SPT = (MAXEL / 2 + NT - 1) / NT;
@@ -278,7 +291,7 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         }
         __syncthreads();
     }
-    fft_lines<C, NT, MAXEL, false, true>(s, nzh, ny, 1, nzh, twy);
+    fft_lines<C, NT, MAXEL, false, true>(s, nzh, ny, 1, nzh, twy, r.pl_r8);
     C *dst = (C *)r.spec + ((long)c * nx + x) * ny * nzh;
     for (int e = tid; e < ny * nzh; e += NT) dst[e] = s[e];
 }
@@ -293,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
 {
     using C = typename cplx_of<T>::t;
     RVE_PROLOGUE(lb);
-    if (mode == 0 && !iter_active(r.ms)) return;
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
@@ -307,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     const C *srcp = (const C *)r.spec + ((long)c * nx + x) * ny * nzh;
     for (int e = tid; e < ny * nzh; e += NT) s[e] = srcp[e];
     __syncthreads();
-    fft_lines<C, NT, MAXEL, true, true>(s, nzh, ny, 1, nzh, twy);
+    fft_lines<C, NT, MAXEL, true, true>(s, nzh, ny, 1, nzh, twy, r.pl_r8);
     // pack the half spectra of rows 2p (a) and 2p+1 (b) into a full complex
     // row Z = A + iB, A and B Hermitian-extended
     {
@@ -344,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         }
         __syncthreads();
     }
-    fft_lines<C, NT, MAXEL, true, false>(s, ny / 2, nz, 2 * nzh, 1, twz);
+    fft_lines<C, NT, MAXEL, true, false>(s, ny / 2, nz, 2 * nzh, 1, twz, r.pl_r8);
     const T *sr = (const T *)s;
     const long off = (long)c * r.V + (long)x * ny * nz;
     if (mode == 0) {
@@ -463,7 +476,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     RVE_PROLOGUE(lb);
     const int tk = r.tk;
     const MacroState *ms = r.ms;
-    if (mode == 0 && !iter_active(ms)) return;
+    if (mode == 0 ? !iter_active(ms) : !rve_need_skew(ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
@@ -484,7 +497,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
         s[(cc * tk + t) * nx + x] = src[cc * cstride + (long)x * ncolt + col0 + t];
     }
     __syncthreads();
-    if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, nx, 1, twx);
+    if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, nx, 1, twx, r.x_r8);
     const bool first = (ms->iter == 0);
     C *usum = (C *)r.usum;
     const T invV = (T)(1.0 / (double)r.V);
@@ -538,7 +551,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
         }
     }
     __syncthreads();
-    fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, nx, 1, twx);
+    fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, nx, 1, twx, r.x_r8);
     C *dst = (C *)r.spec;
     for (int e = tid; e < ncomp * nx * ncol; e += NT) {
         const int cc = e / (nx * ncol);
@@ -601,14 +614,54 @@ __device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5]
 // the scalar cache or L1; holding them would cost occupancy.
 #define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
 
+// slip systems per trip of the flow-rule loop of the Newton kernel
+#ifndef VOCE_NR_WAVES
+#define VOCE_NR_WAVES 2      // waves per SIMD the Newton kernel is compiled for
+#endif
+#ifndef VOCE_NR_R_LDS
+#define VOCE_NR_R_LDS 1      // 1: stage the voxel's rotation in LDS; 0: re-read it
+#endif
+#ifndef VOCE_NR_SYS_UNROLL
+#define VOCE_NR_SYS_UNROLL 1
+#endif
+#define VOCE_PRAGMA_(x) _Pragma(#x)
+#define VOCE_PRAGMA(x) VOCE_PRAGMA_(x)
+
+// Read-only views in the constant address space: the compiler may then use
+// scalar loads for wave-uniform addresses (material constants, C0) even in
+// kernels that store to other global memory.  Only for data that no kernel
+// writes while the view is in use.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+
+#ifndef VOCE_IPOW_MODE
+#define VOCE_IPOW_MODE 0
+#endif
+// x^n for 0 <= n < 64 by binary powering; n is wave-uniform
 __device__ __forceinline__ double ipow(double x, int n)
 {
     double r = 1.0;
-    while (n) {
-        if (n & 1) r *= x;
+#if VOCE_IPOW_MODE == 0
+    // unrolled bit tests: scalar branches, no loop
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        if ((n >> b) == 0) break;
+        if ((n >> b) & 1) r *= x;
         x *= x;
-        n >>= 1;
     }
+#elif VOCE_IPOW_MODE == 1
+    // branch-free: every bit, selects
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double t = r * x;
+        r = ((n >> b) & 1) ? t : r;
+        x *= x;
+    }
+#else
+    (void)n;
+    const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+    r = x16 * x2 * x;
+#endif
     return r;
 }
 
@@ -707,235 +760,6 @@ __device__ __forceinline__ double Dget(const double D[15], int i, int j)
     return D[a * 5 - (a * (a - 1)) / 2 + (b - a)];
 }
 
-// Crout LU with implicit row scaling and partial pivoting
-// (LU_dcmp, kinematics.c:282-354) applied to J x = b; b is permuted along.
-// Returns 0, or 35 for a row whose largest entry is below 1e-5.
-__device__ __forceinline__ int lu_solve6(double a[6][6], double b[6])
-{
-    double vv[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double big = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) big = fmax(big, fabs(a[i][j]));
-        if (big < 1E-5) return 35;
-        vv[i] = 1.0 / big;
-    }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-#pragma unroll
-        for (int i = 0; i < j; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-        }
-        double big = 0.0;
-        int imax = j;
-#pragma unroll
-        for (int i = j; i < 6; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-            const double dum = vv[i] * fabs(sum);
-            if (dum >= big) { big = dum; imax = i; }
-        }
-#pragma unroll
-        for (int i = j + 1; i < 6; ++i) {
-            if (imax == i) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    const double t = a[i][k];
-                    a[i][k] = a[j][k];
-                    a[j][k] = t;
-                }
-                vv[i] = vv[j];
-                const double tb = b[i];
-                b[i] = b[j];
-                b[j] = tb;
-            }
-        }
-        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
-        if (j != 5) {
-            const double dum = 1.0 / a[j][j];
-#pragma unroll
-            for (int i = j + 1; i < 6; ++i) a[i][j] *= dum;
-        }
-    }
-#pragma unroll
-    for (int i = 1; i < 6; ++i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum;
-    }
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = i + 1; k < 6; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum / a[i][i];
-    }
-    return 0;
-}
-
-// The same factorisation and solve, organised for a wavefront: the pivot
-// criterion |a_ij| / max_k |a_ik| (implicit scaling) is compared by cross
-// multiplication instead of through reciprocals, row exchanges run only if
-// some lane of the wave needs one (the Newton Jacobian I + C0 (S + dt D) is
-// diagonally dominant, so in practice none does), and the back substitution
-// reuses the pivot reciprocals.  Same pivots as lu_solve6 except on exact
-// ties of the rounded criterion.
-__device__ __forceinline__ int lu_solve6_wave(double a[6][6], double b[6])
-{
-    double big[6];
-    int bad = 0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double m = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) m = fmax(m, fabs(a[i][j]));
-        big[i] = m;
-        bad |= (m < 1E-5);
-    }
-    if (bad) return 35;
-    double inv[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-#pragma unroll
-        for (int i = 0; i < j; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-        }
-        double bn = 0.0, bd = 1.0;
-        int imax = j;
-#pragma unroll
-        for (int i = j; i < 6; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-            // |sum| / big_i >= bn / bd
-            if (fabs(sum) * bd >= bn * big[i]) { bn = fabs(sum); bd = big[i]; imax = i; }
-        }
-        if (__any(imax != j)) {
-#pragma unroll
-            for (int i = j + 1; i < 6; ++i) {
-                if (imax == i) {
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        const double t = a[i][k];
-                        a[i][k] = a[j][k];
-                        a[j][k] = t;
-                    }
-                    big[i] = big[j];
-                    const double tb = b[i];
-                    b[i] = b[j];
-                    b[j] = tb;
-                }
-            }
-        }
-        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
-        inv[j] = 1.0 / a[j][j];
-        if (j != 5) {
-#pragma unroll
-            for (int i = j + 1; i < 6; ++i) a[i][j] *= inv[j];
-        }
-    }
-#pragma unroll
-    for (int i = 1; i < 6; ++i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum;
-    }
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = i + 1; k < 6; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum * inv[i];
-    }
-    return 0;
-}
-
-// 5x5 version of lu_solve6_wave (same pivot rule and singular-row check).
-__device__ __forceinline__ int lu_solve5_wave(double a[5][5], double b[6])
-{
-    double big[5];
-    int bad = 0;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double m = 0.0;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) m = fmax(m, fabs(a[i][j]));
-        big[i] = m;
-        bad |= (m < 1E-5);
-    }
-    if (bad) return 35;
-    double inv[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-#pragma unroll
-        for (int i = 0; i < j; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < i; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-        }
-        double bn = 0.0, bd = 1.0;
-        int imax = j;
-#pragma unroll
-        for (int i = j; i < 5; ++i) {
-            double sum = a[i][j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) sum -= a[i][k] * a[k][j];
-            a[i][j] = sum;
-            if (fabs(sum) * bd >= bn * big[i]) { bn = fabs(sum); bd = big[i]; imax = i; }
-        }
-        if (__any(imax != j)) {
-#pragma unroll
-            for (int i = j + 1; i < 5; ++i) {
-                if (imax == i) {
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) {
-                        const double t = a[i][k];
-                        a[i][k] = a[j][k];
-                        a[j][k] = t;
-                    }
-                    big[i] = big[j];
-                    const double tb = b[i];
-                    b[i] = b[j];
-                    b[j] = tb;
-                }
-            }
-        }
-        if (fabs(a[j][j]) < 1E-5) a[j][j] = 1.0E-20;
-        inv[j] = 1.0 / a[j][j];
-        if (j != 4) {
-#pragma unroll
-            for (int i = j + 1; i < 5; ++i) a[i][j] *= inv[j];
-        }
-    }
-#pragma unroll
-    for (int i = 1; i < 5; ++i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum;
-    }
-#pragma unroll
-    for (int i = 4; i >= 0; --i) {
-        double sum = b[i];
-#pragma unroll
-        for (int k = i + 1; k < 5; ++k) sum -= a[i][k] * b[k];
-        b[i] = sum * inv[i];
-    }
-    return 0;
-}
 
 template <int NT, int NV>
 __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *out)
@@ -1111,10 +935,12 @@ __device__ void finalize_iteration(MacroState *ms, const double *tot)
     ms->s_vm1 = s_vm1;
     ms->Err_e = Err_e;
     ms->Err_s = Err_s;
-    if (it < VOCE_MAXIT_LOG) {
-        ms->log_err_e[it] = Err_e;
-        ms->log_err_s[it] = Err_s;
-        ms->log_svm[it] = s_vm;
+    if (it < ms->log_cap) {
+        double *lg = ms->logs;
+        const int cap = ms->log_cap;
+        lg[it] = Err_e;
+        lg[cap + it] = Err_s;
+        lg[2 * cap + it] = s_vm;
     }
     ms->iter = it + 1;
     ms->iter_total = itot + 1;
@@ -1142,11 +968,67 @@ __device__ void finalize_iteration(MacroState *ms, const double *tot)
 // held whole.
 //
 // Partials per block: [0] sum|sigma-lambda|, [1] sum|eps_tot-eps|,
-// [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1.  The last
-// block to finish sums all partials in block order and runs the iteration
-// finaliser, so one launch covers UpdateStress, the error Allreduces and
-// get_smacro.  UNI: a single crystalline phase (uniform material constants).
+// [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1, summed in
+// block order by k_fin_iter.  UNI: a single crystalline phase (uniform
+// material constants).
 // ------------------------------------------------------------------
+
+// 1/d: hardware reciprocal estimate refined by two Newton steps
+// r <- r + r (1 - d r), which square the relative error each time.
+__device__ __forceinline__ double rcp_refined(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Solve (K_c + dt D) x = g for symmetric positive definite K_c (6x6, row
+// major) plus dt D (D packed upper 5x5, zero 6th row/column) by LDL^T.
+// Returns 35 (the reference's singular-matrix code) if a pivot is not
+// positive, which a positive definite matrix never gives.
+__device__ __forceinline__ int ldlt_solve6(const double *Kc, const double D[15], double dt, const double g[6],
+                                           double x[6])
+{
+    double L[6][6], Dd[6], iD[6];
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double v[6];
+#pragma unroll
+        for (int k = 0; k < j; ++k) v[k] = L[j][k] * Dd[k];
+        double d = Kc[j * 6 + j] + (j < 5 ? dt * Dget(D, j, j) : 0.0);
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= L[j][k] * v[k];
+        bad |= !(d > 0.0);
+        Dd[j] = d;
+        iD[j] = rcp_refined(d);
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double t = Kc[i * 6 + j] + (i < 5 && j < 5 ? dt * Dget(D, i, j) : 0.0);
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= L[i][k] * v[k];
+            L[i][j] = t * iD[j];
+        }
+    }
+    double y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double t = g[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+        y[i] = t;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double t = y[i] * iD[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) t -= L[k][i] * x[k];
+        x[i] = t;
+    }
+    return bad ? 35 : 0;
+}
 
 // B-basis (b1..b5) components of R P R^T for a symmetric crystal-frame
 // tensor P given as (P00, P11, P22, P12, P02, P01)
@@ -1175,7 +1057,7 @@ __device__ __forceinline__ void rotate_sym_b5(const double R[9], const double *p
 }
 
 template <int NT, bool UNI, bool TWO>
-__global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     RVE_PROLOGUE(lb);
     MacroState *ms = r.ms;
@@ -1184,8 +1066,10 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
     // (component-major: conflict-free), keeping the register file for the loop
     __shared__ double lds_w[6][NT];     // A^-1 c
     __shared__ double lds_old[6][NT];   // stress at the last flow-rule evaluation
+#if VOCE_NR_R_LDS
     __shared__ double lds_R[9][NT];     // TranMat of the voxel
-    __shared__ double lds_cr[TWO ? 24 : 12][NT];   // forward (and backward) CRSS
+#endif
+    __shared__ double lds_ic[TWO ? 24 : 12][NT];   // 1 / CRSS, forward (and backward)
     const long V = r.V;
     const long v = (long)lb * NT + threadIdx.x;
     double acc[14];
@@ -1201,18 +1085,23 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
                 r.P6[i * V + v] = 0.0;
             }
         } else {
-            const PhaseConst *pc = r.phase + (UNI ? 0 : (ph - 1));
+            const cptr<PhaseConst> pc = as_const(r.phase + (UNI ? 0 : (ph - 1)));
             const int nsys = pc->nsys;
-            const double dt = ms->dt;
-            const double *C0 = ms->C066;
+            const cptr<MacroState> msc = as_const(ms);
+            const double dt = msc->dt;
+            const cptr<double> C0 = msc->C066;
             const double *tab = r.SLOC + (long)ori * VOCE_OTAB;   // S_local, A^-1, A^-1 C0
+#if VOCE_NR_R_LDS
 #pragma unroll
             for (int i = 0; i < 9; ++i) lds_R[i][threadIdx.x] = r.R[i * V + v];
+#endif
+            // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
+            // it would form per evaluation)
 #pragma unroll
             for (int q = 0; q < 12; ++q) {
                 if (q < nsys) {
-                    lds_cr[q][threadIdx.x] = r.CRSS[q * V + v];
-                    if (TWO) lds_cr[TWO ? 12 + q : q][threadIdx.x] = r.CRSSB[q * V + v];
+                    lds_ic[q][threadIdx.x] = 1.0 / r.CRSS[q * V + v];
+                    if (TWO) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / r.CRSSB[q * V + v];
                 }
             }
             double sig[6], signorm = 0.0;
@@ -1224,20 +1113,15 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
                     de[i] = r.E6[i * V + v] - (i < 5 ? r.EP5[i * V + v] : 0.0);
                     signorm += sig[i] * sig[i];
                 }
-                double c[6];
+                // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
+                // part of the compliance-form residual below, kept in LDS (a
+                // per-lane register extension)
+                const cptr<double> S0b = msc->S066;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    double t = sig[i];
+                    double t = -de[i];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) t += C0[i * 6 + j] * de[j];
-                    c[i] = t;
-                }
-                // w = A^-1 c, kept in LDS (a per-lane register extension)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) t += tab[36 + i * 6 + j] * c[j];
+                    for (int j = 0; j < 6; ++j) t -= S0b[i * 6 + j] * sig[j];
                     lds_w[i][threadIdx.x] = t;
                 }
             }
@@ -1253,8 +1137,13 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
 #pragma unroll
                 for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
                 double Rm[9];
+#if VOCE_NR_R_LDS
 #pragma unroll
                 for (int i = 0; i < 9; ++i) Rm[i] = lds_R[i][threadIdx.x];
+#else
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];   // re-read (L2) each Newton step
+#endif
                 double T[9];
                 b6_to_t(sig, T);
                 // StrainRate_eval (constitutive.c:4-72): edot and the
@@ -1270,11 +1159,11 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
                 // slip vectors of system s + 1 are loaded while s is computed
                 double bnn[3] = {pc->bn[0][0], pc->bn[0][1], pc->bn[0][2]};
                 double bbn[3] = {pc->bb[0][0], pc->bb[0][1], pc->bb[0][2]};
-#pragma unroll 1
+                VOCE_PRAGMA(unroll VOCE_NR_SYS_UNROLL)
                 for (int s = 0; s < nsys; ++s) {
                     RELOAD_BARRIER();   // slip vectors are re-read from the scalar cache
-                    const double cfs = lds_cr[s][threadIdx.x];
-                    const double cbs = TWO ? lds_cr[TWO ? 12 + s : s][threadIdx.x] : cfs;
+                    const double icf = lds_ic[s][threadIdx.x];
+                    const double icb = TWO ? lds_ic[TWO ? 12 + s : s][threadIdx.x] : icf;
                     const double bn[3] = {bnn[0], bnn[1], bnn[2]}, bb[3] = {bbn[0], bbn[1], bbn[2]};
                     if (s + 1 < nsys) {
 #pragma unroll
@@ -1293,8 +1182,7 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
                         rss0 += n[i] * (T[i * 3 + 0] * bv[0] + T[i * 3 + 1] * bv[1] + T[i * 3 + 2] * bv[2]);
-                    const double tc = (rss0 < 0) ? cbs : cfs;
-                    const double itc = 1.0 / tc;
+                    const double itc = (rss0 < 0) ? icb : icf;
                     const double rr = rss0 * itc;
                     double pn1, pn;
                     if (intn) {
@@ -1328,45 +1216,26 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
                     }
                 }
                 RELOAD_BARRIER();
-                // Newton step x = J^-1 R with J = A M, M = I + dt (A^-1 C0) D:
-                // z = A^-1 R = sigma - w + dt (A^-1 C0) edot, M x = z.  M is the
-                // identity in its 6th column, so rows 0..4 are a 5x5 system and
-                // x_5 follows from row 5.
-                const double *Bm = tab + 72;   // A^-1 C0
-                double z[6], M[5][5];
+                // Newton step x = J^-1 R (evolution.c:70-101) with
+                // J = I + C0 (S_local + dt D) and R = sigma - lambda + C0 (eps_tot - eps),
+                // eps_tot = eps_p_t + dt edot + S_local sigma.  Both carry the left
+                // factor C0 = C066: J = C0 K with K = K_c + dt D, K_c = S0 + S_local,
+                // and C0^-1 R = g = K_c sigma + w + dt edot.  K is symmetric positive
+                // definite (two compliances plus the PSD flow-rule Jacobian), so
+                // x = K^-1 g is an LDL^T solve without pivoting.
+                const double *Kc = tab + 36;
+                double g[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    double t = sig[i] - lds_w[i][threadIdx.x];
+                    double t = lds_w[i][threadIdx.x] + (i < 5 ? dt * ed[i] : 0.0);
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) t += dt * Bm[i * 6 + k] * ed[k];
-                    z[i] = t;
+                    for (int j = 0; j < 6; ++j) t += Kc[i * 6 + j] * sig[j];
+                    g[i] = t;
                 }
-#pragma unroll
-                for (int i = 0; i < 5; ++i)
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) {
-                        double t = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 5; ++k) t += Bm[i * 6 + k] * Dget(D, k, j);
-                        M[i][j] = (double)(i == j) + dt * t;
-                    }
                 double x[6];
-                if (lu_solve5_wave(M, z)) {
+                if (ldlt_solve6(Kc, D, dt, g, x)) {
                     ms->error = 35;
                     break;
-                }
-                {
-                    double t5 = z[5];
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) {
-                        double t = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 5; ++k) t += Bm[5 * 6 + k] * Dget(D, k, j);
-                        t5 -= dt * t * z[j];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 5; ++i) x[i] = z[i];
-                    x[5] = t5;
                 }
                 double cn = 0.0;
 #pragma unroll
@@ -1406,19 +1275,23 @@ __global__ __launch_bounds__(NT) void k_nr(const RveDev *__restrict__ rtab, cons
         }
     }
     block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
-    // last block finalises the iteration
-    __shared__ int amlast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amlast = (atomicAdd(&ms->ticket, 1) == r.nblk_vox - 1);
-    __syncthreads();
-    if (!amlast) return;
+}
+
+// Iteration finaliser: sums the Newton kernel's block partials in block
+// order and runs the error norms, get_smacro and the convergence test.  A
+// kernel of its own (one block per RVE): the kernel boundary makes the
+// partials visible without a device-scope fence in every Newton block,
+// which would write back and invalidate the L2 each time.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fin_iter(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
+{
+    RVE_PROLOGUE(lb);
+    (void)lb;
+    MacroState *ms = r.ms;
+    if (!iter_active(ms)) return;
     __shared__ double tot[16];
     sum_partials<NT, 14>(r.part, r.nblk_vox, tot);
-    if (threadIdx.x == 0) {
-        ms->ticket = 0;
-        finalize_iteration(ms, tot);
-    }
+    if (threadIdx.x == 0) finalize_iteration(ms, tot);
 }
 
 // increment start (evolution.c:306-340)
@@ -1427,9 +1300,23 @@ __global__ void k_begin_increment(const RveDev *__restrict__ rtab, const int2 *_
     RVE_PROLOGUE(lb);
     (void)lb;
     MacroState *ms = r.ms;
+    if (ms->vg_pending) {
+        // a new macroscopic velocity gradient for this increment: target
+        // DisGradAvg_t + L dt (the predictor of evolution.c:460 with the new L)
+        const double dt = ms->dt;
+        for (int i = 0; i < 9; ++i) {
+            const int t = (i % 3) * 3 + i / 3;
+            ms->Udot[i] = ms->vg_next[i];
+            ms->Udot_a[i] = (ms->vg_next[i] - ms->vg_next[t]) / 2.;
+            ms->DisGradAvg[i] = ms->DisGradAvg_t[i] + ms->vg_next[i] * dt;
+        }
+        double sy[9];
+        for (int i = 0; i < 9; ++i) sy[i] = (ms->vg_next[i] + ms->vg_next[(i % 3) * 3 + i / 3]) / 2.;
+        sym_to_b6(sy, ms->udot_sym6);
+        ms->vg_pending = 0;
+    }
     ms->istep += 1;
     ms->iter = 0;
-    ms->ticket = 0;
     for (int i = 0; i < 9; ++i) {
         ms->dD[i] = 0.0;
         ms->acum[i] = 0.0;
@@ -1488,10 +1375,11 @@ __device__ void orientation(double tran[9], const double rot_in[9])
 // block: [0..4] sum Eps (B basis), [5..9] sum Edot.
 // ------------------------------------------------------------------
 template <int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int rot_on)
+__global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     RVE_PROLOGUE(lb);
     const MacroState *ms = r.ms;
+    const bool rot_on = rve_need_skew(ms);
     const long V = r.V;
     const long v = (long)lb * NT + threadIdx.x;
     double acc[10];
@@ -1755,11 +1643,63 @@ __global__ void k_init_sig(const RveDev *__restrict__ rtab, const double *sig6_b
     for (int i = 0; i < 6; ++i) r.S6[i * V + v] = (ori == VOCE_GAS) ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
 }
 
-constexpr int NT_VOX = 256;
+#ifndef VOCE_NT_VOX
+#define VOCE_NT_VOX 256
+#endif
+constexpr int NT_VOX = VOCE_NT_VOX;   // threads per block of the per-voxel kernels
 constexpr int NT_FIN = 256;
+// Launch-shape variants (threads per block, largest tile in complex
+// elements).  Plane kernels: ny * (nz/2+1) <= MAXEL.  x-pass kernel:
+// 6 * tk * nx <= MAXEL.  Which one runs for a grid size is tuned.
+constexpr int NT_PL_T = 64, MAXEL_PL_T = 576;     // planes up to 32 x 32
 constexpr int NT_PL_S = 256, MAXEL_PL_S = 2112;   // planes up to 64 x 64
 constexpr int NT_PL_L = 512, MAXEL_PL_L = 8320;   // planes up to 128 x 128
 constexpr int NT_X = 256, MAXEL_X = 3072;
+constexpr int NT_XB = 128, MAXEL_XB = 1536;
+constexpr int NT_XC = 64, MAXEL_XC = 768;
+
+template <typename T>
+void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
+{
+    switch (L->plv) {
+    case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_plane6), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_plane6); break;
+    case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6); break;
+    default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6); break;
+    }
+}
+
+template <typename T>
+void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
+{
+    const int n = mode ? L->n_plane3 : L->n_plane6;
+    const int2 *map = mode ? L->map_plane3 : L->map_plane6;
+    switch (L->plv) {
+    case 0: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_T, MAXEL_PL_T>), dim3(n), dim3(NT_PL_T), L->pl_lds, st, L->tab, map, mode); break;
+    case 1: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(n), dim3(NT_PL_S), L->pl_lds, st, L->tab, map, mode); break;
+    default: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
+    }
+}
+
+template <typename T>
+void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
+{
+    switch (L->xv) {
+    case 0: hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
+    case 1: hipLaunchKernelGGL((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
+    default: hipLaunchKernelGGL((k_xgreen<T, NT_XC, MAXEL_XC>), dim3(L->n_x), dim3(NT_XC), L->x_lds, st, L->tab, L->map_x, mode); break;
+    }
+}
+
+void launch_nr(const LaunchSet *L, hipStream_t st)
+{
+    if (L->uniform_phase) {
+        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+    } else {
+        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+    }
+}
 
 template <typename T>
 hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
@@ -1770,28 +1710,15 @@ hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
         if (ev) (void)hipEventRecord(ev[evi++], st);             \
     } while (0)
     MARK();
-    if (L->large) {
-        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6);
-    } else {
-        hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6);
-    }
+    launch_plane_fwd<T>(L, st);
     MARK();
-    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, 0);
+    launch_xgreen<T>(L, st, 0);
     MARK();
-    if (L->large) {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6, 0);
-    } else {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6, 0);
-    }
+    launch_plane_inv<T>(L, st, 0);
     MARK();
-    if (L->uniform_phase) {
-        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-    } else {
-        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-    }
+    launch_nr(L, st);
     MARK();
+    hipLaunchKernelGGL((k_fin_iter<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, st, L->tab, L->map_one);
     MARK();
 #undef MARK
     return hipGetLastError();
@@ -1800,12 +1727,8 @@ hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
 template <typename T>
 hipError_t launch_skew_t(const LaunchSet *L, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, 1);
-    if (L->large) {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane3), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane3, 1);
-    } else {
-        hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane3), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane3, 1);
-    }
+    launch_xgreen<T>(L, st, 1);
+    launch_plane_inv<T>(L, st, 1);
     return hipGetLastError();
 }
 
@@ -1813,9 +1736,11 @@ template <typename T>
 hipError_t set_lds_limits()
 {
     const int big = 160 * 1024;
-    const void *fns[] = {(const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, (const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>,
+    const void *fns[] = {(const void *)k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>, (const void *)k_plane_inv<T, NT_PL_T, MAXEL_PL_T>,
                          (const void *)k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>, (const void *)k_plane_inv<T, NT_PL_S, MAXEL_PL_S>,
-                         (const void *)k_xgreen<T, NT_X, MAXEL_X>};
+                         (const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, (const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>,
+                         (const void *)k_xgreen<T, NT_X, MAXEL_X>, (const void *)k_xgreen<T, NT_XB, MAXEL_XB>,
+                         (const void *)k_xgreen<T, NT_XC, MAXEL_XC>};
     for (const void *f : fns) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, big);
         if (e != hipSuccess) return e;
@@ -1842,21 +1767,37 @@ extern "C" int voce_grid_supported(int n)
     return m == 1;
 }
 
-extern "C" int voce_xgreen_tile(int nx)
+extern "C" int voce_vox_block(void) { return NT_VOX; }
+
+static const int PL_MAXEL[3] = {MAXEL_PL_T, MAXEL_PL_S, MAXEL_PL_L};
+static const int X_MAXEL[3] = {MAXEL_X, MAXEL_XB, MAXEL_XC};
+static const size_t LDS_MAX = 160 * 1024;
+
+// Largest x-pass tile (columns per block, <= 16) for variant xv.
+extern "C" int voce_xgreen_tile(int nx, int xv)
 {
-    int tk = MAXEL_X / (6 * nx);
+    int tk = X_MAXEL[xv] / (6 * nx);
     if (tk > 16) tk = 16;
-    if (tk < 1) tk = 1;
-    return tk;
+    return tk;   // 0: the variant cannot hold one column
 }
 
-// Launch classes of one RVE (plane-kernel size class and dynamic LDS).
-extern "C" void voce_launch_shape(const RveDev *r, int *large, size_t *pl_lds, size_t *x_lds)
+// Dynamic LDS of the plane and x-pass kernels of one RVE under its tile
+// choices; 0 when a choice does not fit (tile larger than the variant's
+// MAXEL or LDS beyond 160 KB).
+extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 {
+    if (plv < 0 || plv > 2 || r->ny * r->nzh > PL_MAXEL[plv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
-    *large = (r->ny * r->nzh > MAXEL_PL_S);
-    *pl_lds = (size_t)(r->ny * r->nzh + r->ny + r->nz) * cb;
-    *x_lds = (size_t)(6 * r->tk * r->nx + r->nx) * cb;
+    const size_t b = (size_t)(r->ny * r->nzh + r->ny + r->nz) * cb;
+    return b <= LDS_MAX ? b : 0;
+}
+
+extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
+{
+    if (xv < 0 || xv > 2 || tk < 1 || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
+    const size_t cb = r->fft_f32 ? 8 : 16;
+    const size_t b = (size_t)(6 * tk * r->nx + r->nx) * cb;
+    return b <= LDS_MAX ? b : 0;
 }
 
 extern "C" int voce_launch_begin_increment(const LaunchSet *L, void *stream)
@@ -1878,8 +1819,8 @@ extern "C" int voce_launch_closeout(const LaunchSet *L, void *stream, int need_s
     hipError_t e = hipSuccess;
     if (need_skew) e = L->fft_f32 ? launch_skew_t<float>(L, st) : launch_skew_t<double>(L, st);
     if (e != hipSuccess) return (int)e;
-    if (L->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox, need_skew);
-    else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox, need_skew);
+    if (L->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+    else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, st, L->tab, L->map_one);
     return (int)hipGetLastError();
 }
