@@ -1,0 +1,88 @@
+/* voce_batch.h -- many RVEs on one GPU, launch tuning, request serving.
+ *
+ * No counterpart in the reference, which advances one RVE per MPI job
+ * (evp.c:7-32).  Every member of a batch is a full voce_ctx: the
+ * single-RVE entry points of voce.h (voce_get_field, voce_write_fields,
+ * voce_set_diagnostics) work on it unchanged.
+ */
+#ifndef VOCE_BATCH_H
+#define VOCE_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "voce.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- FFT launch tuning ------------------------------------------------ */
+
+/* Launch choices of the three FFT kernels for one grid size and precision:
+ * plane-kernel variant (0: 64 threads / planes <= 32x32, 1: 256 / <= 64x64,
+ * 2: 512 / <= 128x128), x-pass variant (0: 256 threads, 1: 128, 2: 64),
+ * x-pass tile (columns per block), and whether radix-8 stages are used in
+ * the plane and x passes.  us_* are the measured launch times (microseconds)
+ * of the chosen plane pair (forward + inverse) and x pass. */
+typedef struct voce_tune_choice {
+    int plv, xv, tk, pl_r8, x_r8;
+    double us_plane, us_x;
+} voce_tune_choice;
+
+/* Time every valid choice on this context's own buffers (state is saved
+ * and restored; call between increments), keep the fastest, apply it to
+ * the context and return it in *out (may be NULL).  Results are cached per
+ * (nx, ny, nz, precision) for the process; with VOCE_TUNE_FILE set in the
+ * environment they are also read from and appended to that text file. */
+int voce_autotune(voce_ctx *ctx, voce_tune_choice *out);
+/* Apply a known choice (e.g. from a file) without timing. */
+int voce_apply_tuning(voce_ctx *ctx, const voce_tune_choice *choice);
+
+/* ---- batches ---------------------------------------------------------- */
+
+typedef struct voce_batch voce_batch;
+
+/* n RVEs, one voce_setup each (grid sizes may differ; all must share one
+ * precision).  tune = 1 autotunes each distinct grid size once.  Returns
+ * 0 or the first member's error code. */
+int voce_batch_create(const voce_setup *setups, int n, int tune, voce_batch **out);
+/* Advance every member by one strain increment.  Iterations of all members
+ * run in the same kernels (block maps over the members), replayed from a
+ * captured HIP graph; converged members drop out of the work while the
+ * rest iterate.  reports: n entries, or NULL. */
+int voce_batch_increment(voce_batch *b, voce_incr_report *reports);
+int voce_batch_size(const voce_batch *b);
+/* Member i as a context (owned by the batch: do not destroy). */
+voce_ctx *voce_batch_member(voce_batch *b, int i);
+/* out[0..7]: members, groups (distinct grid sizes), total voxels, graph
+ * replays, graph instantiations, FFT iterations launched, last increment's
+ * largest iteration count, bytes of device memory held */
+int voce_batch_info(const voce_batch *b, long out[8]);
+void voce_batch_destroy(voce_batch *b);
+
+/* ---- macroscopic velocity gradient per increment ------------------------ */
+
+/* Velocity gradient L (row-major 3x3, 1/s) for the context's next
+ * increment: that increment targets DisGradAvg_t + L dt (the predictor of
+ * evolution.c:460 with the new L) and uses skew(L) as the applied spin. */
+int voce_set_velgrad(voce_ctx *ctx, const double L[9]);
+
+/* ---- synthetic microstructures ------------------------------------------ */
+
+/* Periodic Voronoi grains on an nx*ny*nz grid (x slowest): seeds uniform in
+ * [0,N)^3 from SplitMix64(seed), voxel centres at (i+0.5, j+0.5, k+0.5),
+ * minimum-image distance, ties to the lowest seed id; then one Bunge triple
+ * per grain, uniform on SO(3) (phi1 = 360u, Phi = acos(2u-1)*180/pi,
+ * phi2 = 360u), rounded as "%.6f" prints it.  grain: V ints (1-based);
+ * euler_by_grain: 3 * n_grains doubles (degrees). */
+int voce_voronoi(const int dims[3], int n_grains, uint64_t seed, int *grain, double *euler_by_grain);
+/* voce_load_case with the mesh generated instead of read: dims (overriding
+ * the namelist's CellDim) and voce_voronoi(n_grains, seed), all phase 1. */
+int voce_load_case_voronoi(const char *input_path, const char *base_dir, const int dims[3], int n_grains,
+                           uint64_t seed, voce_case *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -143,3 +143,56 @@ def test_voronoi_generator_properties():
         d = p - seeds
         d -= 8 * np.round(d / 8)
         assert np.argmin((d * d).sum(1)) + 1 == g[v]
+
+
+def _voronoi_brute(dims, n, seed):
+    """Plain-loop restatement of the generator's definition (seeds, then
+    Euler triples, then nearest seed by minimum image, ties to the lower id)."""
+    import math
+    from voce_amd.inputs import SplitMix64
+    nx, ny, nz = dims
+    rng = SplitMix64(seed)
+    seeds = [(rng.uniform() * nx, rng.uniform() * ny, rng.uniform() * nz) for _ in range(n)]
+    eul = []
+    for _ in range(n):
+        a = 360.0 * rng.uniform()
+        b = math.acos(2.0 * rng.uniform() - 1.0) * 180.0 / math.pi
+        c = 360.0 * rng.uniform()
+        eul.append([float("%.6f" % a), float("%.6f" % b), float("%.6f" % c)])
+    L = (nx, ny, nz)
+    gid = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                p = (i + 0.5, j + 0.5, k + 0.5)
+                best, arg = math.inf, 0
+                for q in range(n):
+                    d = [p[t] - seeds[q][t] for t in range(3)]
+                    d = [d[t] - L[t] * round(d[t] / L[t]) for t in range(3)]   # round: ties to even
+                    r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+                    if r2 < best:
+                        best, arg = r2, q
+                gid.append(arg + 1)
+    return np.array(gid, dtype=np.int32), np.array(eul)
+
+
+@pytest.mark.parametrize("dims,n,seed", [((6, 8, 4), 5, 7), ((12, 6, 10), 17, 123456789), ((16, 16, 16), 8, 1)])
+def test_voronoi_generator_matches_definition(dims, n, seed):
+    g, e = inputs.voronoi_grains(dims, n, seed)
+    gb, eb = _voronoi_brute(dims, n, seed)
+    assert np.array_equal(g, gb)
+    assert np.array_equal(e, eb)
+
+
+def test_synthetic_case_loader(tmp_path):
+    """voce_load_case_voronoi == voce_load_case on the mesh file the same generator writes."""
+    import ctypes as C
+    c = inputs.Case(dims=(8, 6, 4), n_grains=4, seed=11, n_steps=2)
+    g, e = inputs.voronoi_grains(c.dims, c.n_grains, c.seed)
+    inputs.write_case(str(tmp_path), c, grain=g, euler=e)
+    ref = voce_amd.load_case(str(tmp_path / "input"), str(tmp_path))
+    syn = voce_amd.load_case_voronoi(str(tmp_path / "input"), str(tmp_path), (8, 6, 4), 4, 11)
+    e1, g1, p1 = ref.arrays()
+    e2, g2, p2 = syn.arrays()
+    assert np.array_equal(g1, g2) and np.array_equal(e1, e2) and np.array_equal(p1, p2)
+    assert list(syn.setup.dims) == [8, 6, 4]

@@ -87,7 +87,11 @@ EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_fi
            "voce_set_diagnostics", "voce_ctx_nvox", "voce_ctx_destroy", "voce_strerror",
            "voce_load_case", "voce_case_free", "voce_write_fields",
            # include/voce_prof.h
-           "voce_prof_enable", "voce_prof_read", "voce_ctx_info"]
+           "voce_prof_enable", "voce_prof_read", "voce_ctx_info",
+           # include/voce_batch.h
+           "voce_autotune", "voce_apply_tuning", "voce_batch_create", "voce_batch_increment",
+           "voce_batch_size", "voce_batch_member", "voce_batch_info", "voce_batch_destroy",
+           "voce_set_velgrad", "voce_voronoi", "voce_load_case_voronoi"]
 
 PROF_KERNELS = ["plane_fwd", "xgreen", "plane_inv", "nr", "fin_iter"]
 
@@ -120,20 +124,61 @@ def load_library(path: str = LIB_PATH):
     lib.voce_prof_enable.argtypes = [C.c_void_p, C.c_int]
     lib.voce_prof_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     lib.voce_ctx_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+    lib.voce_voronoi.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.voce_load_case_voronoi.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int), C.c_int, C.c_uint64,
+                                           C.POINTER(Case)]
+    lib.voce_autotune.argtypes = [C.c_void_p, C.POINTER(TuneChoice)]
+    lib.voce_apply_tuning.argtypes = [C.c_void_p, C.POINTER(TuneChoice)]
+    lib.voce_set_velgrad.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.voce_batch_create.argtypes = [C.POINTER(Setup), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    lib.voce_batch_increment.argtypes = [C.c_void_p, C.POINTER(Report)]
+    lib.voce_batch_size.argtypes = [C.c_void_p]
+    lib.voce_batch_member.argtypes = [C.c_void_p, C.c_int]
+    lib.voce_batch_member.restype = C.c_void_p
+    lib.voce_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+    lib.voce_batch_destroy.argtypes = [C.c_void_p]
+    lib.voce_batch_destroy.restype = None
     _lib = lib
     return lib
 
 
-class LoadedCase:
-    """A parsed input (namelist + .sx + mesh); owns the C arrays."""
+class TuneChoice(C.Structure):
+    _fields_ = [("plv", C.c_int), ("xv", C.c_int), ("tk", C.c_int), ("pl_r8", C.c_int), ("x_r8", C.c_int),
+                ("us_plane", C.c_double), ("us_x", C.c_double)]
 
-    def __init__(self, path: str, base_dir: str | None = None):
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LoadedCase:
+    """A parsed input (namelist + .sx + mesh); owns the C arrays.  With
+    `voronoi=(dims, n_grains, seed)` the mesh is generated instead of read."""
+
+    def __init__(self, path: str, base_dir: str | None = None, voronoi=None):
         lib = load_library()
         self._c = Case()
-        rc = lib.voce_load_case(path.encode(), base_dir.encode() if base_dir else None, C.byref(self._c))
+        b = base_dir.encode() if base_dir else None
+        if voronoi is None:
+            rc = lib.voce_load_case(path.encode(), b, C.byref(self._c))
+        else:
+            dims, n, seed = voronoi
+            d3 = (C.c_int * 3)(*[int(x) for x in dims])
+            rc = lib.voce_load_case_voronoi(path.encode(), b, d3, int(n), C.c_uint64(int(seed) & ((1 << 64) - 1)),
+                                            C.byref(self._c))
         if rc:
             raise VoceError(rc, "voce_load_case(%s)" % path)
         self._freed = False
+
+    def set_velgrad_full(self, L):
+        """Full-velocity boundary conditions with a general (non-symmetric)
+        macroscopic velocity gradient L (3x3, 1/s)."""
+        s = self._c.setup
+        s.use_full_velgrad = 1
+        for i, x in enumerate(np.asarray(L, dtype=np.float64).ravel()):
+            s.velgrad_full[i] = float(x)
+        for i in range(6):
+            s.velgrad_flag[i] = 1
+            s.stress_flag[i] = 0
 
     @property
     def setup(self) -> Setup:
@@ -173,6 +218,11 @@ class LoadedCase:
 
 def load_case(path: str, base_dir: str | None = None) -> LoadedCase:
     return LoadedCase(path, base_dir)
+
+
+def load_case_voronoi(path: str, base_dir, dims, n_grains: int, seed: int) -> LoadedCase:
+    """Namelist and materials from `path`, mesh from voce_voronoi."""
+    return LoadedCase(path, base_dir, voronoi=(dims, n_grains, seed))
 
 
 def report_dict(r: Report) -> dict:
@@ -231,6 +281,19 @@ class Solver:
         if rc:
             raise VoceError(rc, "voce_write_fields")
 
+    def autotune(self) -> dict:
+        ch = TuneChoice()
+        rc = load_library().voce_autotune(self._h, C.byref(ch))
+        if rc:
+            raise VoceError(rc, "voce_autotune")
+        return ch.as_dict()
+
+    def set_velgrad(self, L):
+        a = (C.c_double * 9)(*[float(x) for x in np.asarray(L, dtype=np.float64).ravel()])
+        rc = load_library().voce_set_velgrad(self._h, a)
+        if rc:
+            raise VoceError(rc, "voce_set_velgrad")
+
     def prof_enable(self, on=True):
         load_library().voce_prof_enable(self._h, int(bool(on)))
 
@@ -242,7 +305,69 @@ class Solver:
 
     def close(self):
         if getattr(self, "_h", None):
-            load_library().voce_ctx_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                load_library().voce_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Member(Solver):
+    """A batch member viewed as a context (owned by its batch)."""
+
+    def __init__(self, handle, nvox, batch):
+        self._h = C.c_void_p(handle)
+        self._borrowed = True
+        self._keep = batch
+        self.nvox = nvox
+
+
+class Batch:
+    """Many RVEs advanced together (include/voce_batch.h)."""
+
+    def __init__(self, cases, precision="fp64", tune=True):
+        lib = load_library()
+        self._cases = list(cases)
+        n = len(self._cases)
+        arr = (Setup * n)()
+        for i, cs in enumerate(self._cases):
+            arr[i] = cs.setup
+            arr[i].precision = MIXED if precision in ("mixed", MIXED) else FP64
+        h = C.c_void_p()
+        rc = lib.voce_batch_create(arr, n, int(bool(tune)), C.byref(h))
+        if rc:
+            raise VoceError(rc, "voce_batch_create")
+        self._h = h
+        self.n = n
+        self.nvox = [cs.nvox for cs in self._cases]
+
+    def increment(self):
+        reps = (Report * self.n)()
+        rc = load_library().voce_batch_increment(self._h, reps)
+        if rc:
+            raise VoceError(rc, "voce_batch_increment")
+        return [report_dict(r) for r in reps]
+
+    def member(self, i) -> Solver:
+        hm = load_library().voce_batch_member(self._h, int(i))
+        if not hm:
+            raise IndexError(i)
+        return _Member(hm, self.nvox[i], self)
+
+    def info(self) -> dict:
+        out = (C.c_long * 8)()
+        load_library().voce_batch_info(self._h, out)
+        keys = ("members", "groups", "voxels", "graph_replays", "graphs", "iterations_launched", "last_max_iter",
+                "device_bytes")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().voce_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -33,15 +33,27 @@ def build(verbose_resources=False):
     obj = os.path.join(HERE, "_obj")
     os.makedirs(obj, exist_ok=True)
     os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
-    k_o = os.path.join(obj, "voce_kernels.o")
-    c_o = os.path.join(obj, "voce_ctx.o")
-    io_o = os.path.join(obj, "voce_io.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
-    run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "voce_kernels.hip"), "-o", k_o])
-    run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "voce_ctx.cpp"), "-o", c_o])
-    run(["gcc", "-O2", "-fPIC", "-std=gnu99", "-Wall", "-c", os.path.join(CSRC, "voce_io.c"), "-o", io_o])
+    objs = []
+    jobs = []
+    for src, flags in (("voce_kernels.hip", extra), ("voce_ctx.cpp", []), ("voce_tune.cpp", []),
+                       ("voce_batch.cpp", []), ("voce_serve.cpp", []), ("voce_gen.cpp", [])):
+        o = os.path.join(obj, os.path.splitext(src)[0] + ".o")
+        objs.append(o)
+        jobs.append([HIPCC] + HIP_FLAGS + flags + ["-c", os.path.join(CSRC, src), "-o", o])
+    io_o = os.path.join(obj, "voce_io.o")
+    objs.append(io_o)
+    jobs.append(["gcc", "-O2", "-fPIC", "-std=gnu99", "-Wall", "-c", os.path.join(CSRC, "voce_io.c"), "-o", io_o])
+    # compile the translation units side by side
+    procs = []
+    for cmd in jobs:
+        print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    bad = [cmd for cmd, p in procs if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
     lib = os.path.join(HERE, "libvoce.so")
-    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, k_o, c_o, io_o, "-o", lib])
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-lpthread", "-o", lib])
     run(["gcc", "-O2", "-std=gnu99", "-Wall", os.path.join(CSRC, "voce_main.c"), "-o",
          os.path.join(HERE, "bin", "voce"), "-L" + HERE, "-lvoce", "-Wl,-rpath,$ORIGIN/.."])
     # calibration kernel for the HBM traffic counters

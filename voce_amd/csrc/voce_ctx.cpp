@@ -16,6 +16,7 @@
 
 #include "../../include/voce.h"
 #include "../../include/voce_prof.h"
+#include "../../include/voce_batch.h"
 #include "voce_internal.h"
 #include "voce_host.h"
 
@@ -1049,5 +1050,19 @@ extern "C" int voce_ctx_info(const voce_ctx *c, long *out)
     out[3] = c->dims[2];
     out[4] = c->precision;
     out[5] = c->two_crss_any;
+    return VOCE_OK;
+}
+
+extern "C" int voce_set_velgrad(voce_ctx *c, const double L[9])
+{
+    if (!c || !L) return VOCE_ERR_ARG;
+    struct { double vg[9]; } p;
+    memcpy(p.vg, L, sizeof(p.vg));
+    const int one = 1;
+    CK(hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_next), p.vg, sizeof(p.vg), hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_pending), &one, sizeof(int), hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipStreamSynchronize(c->stream));
     return VOCE_OK;
 }

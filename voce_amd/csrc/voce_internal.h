@@ -129,6 +129,8 @@ extern "C" {
 int voce_launch_begin_increment(const LaunchSet *L, void *stream);
 int voce_launch_iteration(const LaunchSet *L, void *stream, void *events /* hipEvent_t[6] or NULL */);
 int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew);
+int voce_launch_fft_probe(const LaunchSet *L, void *stream, int which);
+int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, void *stream);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 int voce_kernels_prepare(void);

@@ -13,13 +13,18 @@
  *             voxel, "phi1 Phi phi2 i j k grain phase", x slowest; an index
  *             that disagrees with the record's position is abort 1139.
  *   latent    PlasticInit's 12x12 table (init.c:283-381).
+ *
+ * voce_load_case_voronoi takes the mesh from voce_voronoi (voce_gen.cpp)
+ * instead of a file.
  */
 #include <stddef.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "../../include/voce.h"
+#include "../../include/voce_batch.h"
 
 typedef struct {
     int CellDim[3];
@@ -230,7 +235,23 @@ void voce_case_free(voce_case *c)
     c->phase_id = NULL;
 }
 
+static int load_case_impl(const char *input_path, const char *base_dir, const int *dims_override, int n_grains,
+                          uint64_t seed, voce_case *out);
+
 int voce_load_case(const char *input_path, const char *base_dir, voce_case *out)
+{
+    return load_case_impl(input_path, base_dir, NULL, 0, 0, out);
+}
+
+int voce_load_case_voronoi(const char *input_path, const char *base_dir, const int dims[3], int n_grains,
+                           uint64_t seed, voce_case *out)
+{
+    if (n_grains <= 0) return VOCE_ERR_ARG;
+    return load_case_impl(input_path, base_dir, dims, n_grains, seed, out);
+}
+
+static int load_case_impl(const char *input_path, const char *base_dir, const int *dims_override, int n_grains,
+                          uint64_t seed, voce_case *out)
 {
     namelist_t nm;
     char path[1200];
@@ -241,7 +262,7 @@ int voce_load_case(const char *input_path, const char *base_dir, voce_case *out)
     memset(out, 0, sizeof(*out));
     if ((rc = read_namelist(input_path, &nm, out->namelist_status))) return rc;
     s = &out->setup;
-    memcpy(s->dims, nm.CellDim, sizeof(s->dims));
+    memcpy(s->dims, dims_override ? dims_override : nm.CellDim, sizeof(s->dims));
     if (s->dims[0] <= 0 || s->dims[1] <= 0 || s->dims[2] <= 0) return VOCE_ERR_FILE;
     if (nm.N_phases < 1 || nm.N_phases > VOCE_NPHASE) return VOCE_ERR_FILE;
     s->n_phases = nm.N_phases;
@@ -280,10 +301,33 @@ int voce_load_case(const char *input_path, const char *base_dir, voce_case *out)
         voce_case_free(out);
         return VOCE_ERR_NOMEM;
     }
-    join(path, sizeof(path), base_dir, nm.initial_ms);
-    if ((rc = read_mesh(path, s->dims, out->euler_deg, out->grain, out->phase_id))) {
-        voce_case_free(out);
-        return rc;
+    if (n_grains > 0) {
+        double *eg = (double *)malloc(sizeof(double) * 3 * n_grains);
+        long v;
+        if (!eg) {
+            voce_case_free(out);
+            return VOCE_ERR_NOMEM;
+        }
+        rc = voce_voronoi(s->dims, n_grains, seed, out->grain, eg);
+        if (!rc)
+            for (v = 0; v < V; ++v) {
+                const int g = out->grain[v] - 1;
+                out->euler_deg[3 * v] = eg[3 * g];
+                out->euler_deg[3 * v + 1] = eg[3 * g + 1];
+                out->euler_deg[3 * v + 2] = eg[3 * g + 2];
+                out->phase_id[v] = 1;
+            }
+        free(eg);
+        if (rc) {
+            voce_case_free(out);
+            return rc;
+        }
+    } else {
+        join(path, sizeof(path), base_dir, nm.initial_ms);
+        if ((rc = read_mesh(path, s->dims, out->euler_deg, out->grain, out->phase_id))) {
+            voce_case_free(out);
+            return rc;
+        }
     }
     s->euler_deg = out->euler_deg;
     s->grain = out->grain;

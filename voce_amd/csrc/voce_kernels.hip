@@ -1632,6 +1632,31 @@ __global__ __launch_bounds__(NT) void k_fin_incr(const RveDev *__restrict__ rtab
     ms->dvmp = sqrt(2. / 3. * dvmp);
 }
 
+// Batch convergence poll: out[0] = members still iterating, out[1] = the
+// largest error code among them (0: none).
+__global__ __launch_bounds__(256) void k_count_active(const RveDev *__restrict__ tab, int n, int *out)
+{
+    __shared__ int s_act, s_err;
+    if (threadIdx.x == 0) {
+        s_act = 0;
+        s_err = 0;
+    }
+    __syncthreads();
+    int a = 0, e = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const MacroState *ms = tab[i].ms;
+        a += ms->active != 0;
+        e = max(e, ms->error);
+    }
+    atomicAdd(&s_act, a);
+    atomicMax(&s_err, e);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = s_act;
+        out[1] = s_err;
+    }
+}
+
 // initial fields: S6 from the per-orientation table (InitFields, init.c:486-534)
 __global__ void k_init_sig(const RveDev *__restrict__ rtab, const double *sig6_by_orient)
 {
@@ -1806,6 +1831,23 @@ extern "C" int voce_launch_begin_increment(const LaunchSet *L, void *stream)
     return (int)hipGetLastError();
 }
 
+// One FFT kernel of the iteration alone (0: plane_fwd, 1: xgreen, 2:
+// plane_inv), for timing launch choices.
+extern "C" int voce_launch_fft_probe(const LaunchSet *L, void *stream, int which)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (L->fft_f32) {
+        if (which == 0) launch_plane_fwd<float>(L, st);
+        else if (which == 1) launch_xgreen<float>(L, st, 0);
+        else launch_plane_inv<float>(L, st, 0);
+    } else {
+        if (which == 0) launch_plane_fwd<double>(L, st);
+        else if (which == 1) launch_xgreen<double>(L, st, 0);
+        else launch_plane_inv<double>(L, st, 0);
+    }
+    return (int)hipGetLastError();
+}
+
 extern "C" int voce_launch_iteration(const LaunchSet *L, void *stream, void *events)
 {
     hipStream_t st = (hipStream_t)stream;
@@ -1822,6 +1864,12 @@ extern "C" int voce_launch_closeout(const LaunchSet *L, void *stream, int need_s
     if (L->uniform_phase) hipLaunchKernelGGL((k_closeout<NT_VOX, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     else hipLaunchKernelGGL((k_closeout<NT_VOX, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     hipLaunchKernelGGL((k_fin_incr<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, st, L->tab, L->map_one);
+    return (int)hipGetLastError();
+}
+
+extern "C" int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, void *stream)
+{
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, (hipStream_t)stream, tab, n, out_dev);
     return (int)hipGetLastError();
 }
 

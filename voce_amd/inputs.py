@@ -43,38 +43,19 @@ class SplitMix64:
 
 def voronoi_grains(dims, n_grains: int, seed: int):
     """Return (grain_id[V] 1-based int32, euler_deg[n_grains,3]) for a
-    periodic Voronoi tessellation of an nx*ny*nz grid (x slowest)."""
-    nx, ny, nz = dims
-    rng = SplitMix64(seed)
-    seeds = np.array([[rng.uniform() * nx, rng.uniform() * ny, rng.uniform() * nz]
-                      for _ in range(n_grains)], dtype=np.float64)
-    eul = []
-    for _ in range(n_grains):
-        p1 = 360.0 * rng.uniform()
-        ph = math.acos(2.0 * rng.uniform() - 1.0) * 180.0 / math.pi
-        p2 = 360.0 * rng.uniform()
-        eul.append((p1, ph, p2))
-    eul = np.array(eul, dtype=np.float64)
-    # round to the printed precision so that every reader sees the same angles
-    eul = np.array([[float("%.6f" % a) for a in row] for row in eul])
-    L = np.array([nx, ny, nz], dtype=np.float64)
+    periodic Voronoi tessellation of an nx*ny*nz grid (x slowest), computed
+    by the library's voce_voronoi (include/voce_batch.h)."""
+    import ctypes as C
+    from . import load_library
+    lib = load_library()
+    nx, ny, nz = (int(d) for d in dims)
     gid = np.empty(nx * ny * nz, dtype=np.int32)
-    yy, zz = np.meshgrid(np.arange(ny) + 0.5, np.arange(nz) + 0.5, indexing="ij")
-    yz = np.stack([yy.ravel(), zz.ravel()], axis=1)
-    for ix in range(nx):
-        pts = np.empty((ny * nz, 3))
-        pts[:, 0] = ix + 0.5
-        pts[:, 1:] = yz
-        best = np.full(ny * nz, np.inf)
-        arg = np.zeros(ny * nz, dtype=np.int32)
-        for g in range(n_grains):
-            d = pts - seeds[g]
-            d -= L * np.round(d / L)
-            r2 = np.einsum("ij,ij->i", d, d)
-            better = r2 < best  # strict: ties stay with the lower id
-            best[better] = r2[better]
-            arg[better] = g
-        gid[ix * ny * nz:(ix + 1) * ny * nz] = arg + 1
+    eul = np.empty((n_grains, 3), dtype=np.float64)
+    d3 = (C.c_int * 3)(nx, ny, nz)
+    rc = lib.voce_voronoi(d3, int(n_grains), C.c_uint64(seed & _MASK), gid.ctypes.data_as(C.POINTER(C.c_int)),
+                          eul.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise RuntimeError("voce_voronoi failed: %d" % rc)
     return gid, eul
 
 
@@ -165,8 +146,10 @@ class Case:
 
 
 def _fmt(v):
-    if isinstance(v, float):
-        return repr(v)
+    if isinstance(v, (float, np.floating)):
+        return repr(float(v))
+    if isinstance(v, np.integer):
+        return str(int(v))
     return str(v)
 
 
@@ -198,11 +181,17 @@ def write_namelist(path, c: Case, mesh_name="mesh.txt", sx_name="material.sx"):
     names = ("self", "coplane", "cross", "glissile", "hirth", "lomer")
     for n, v in zip(names, c.latent):
         kv.append((n, "%s 0.0" % _fmt(float(v))))
+    lines = ["%s\t%s" % (k, v) for k, v in kv]
+    for ln in lines:
+        # GetNameList reads 80-byte records (io.c:64): a longer line would be
+        # split and its tail silently lost
+        if len(ln) >= 79:
+            raise ValueError("namelist line longer than the reference reads (79 chars): %r" % ln)
     with open(path, "w") as f:
         f.write("# synthetic case: %s grid, %d grains, seed %d\n" % ("x".join(map(str, c.dims)),
                                                                      c.n_grains, c.seed))
-        for k, v in kv:
-            f.write("%s\t%s\n" % (k, v))
+        for ln in lines:
+            f.write(ln + "\n")
 
 
 def write_case(directory, c: Case, grain=None, euler=None):
@@ -230,3 +219,95 @@ def config_case(n: int) -> Case:
                     velgrad_flag=(1, 1, 1, 1, 1, 1), velgrad=(1.0e-3, 0.0, -1.0e-3, 0.0, 0.0, 0.0),
                     stress_flag=(0, 0, 0, 0, 0, 0))
     raise ValueError(n)
+
+
+# ---------------------------------------------------------------------------
+# Many-RVE workloads
+# ---------------------------------------------------------------------------
+
+def _traceless_sym(rng):
+    a = [2.0 * rng.uniform() - 1.0 for _ in range(6)]
+    D = np.array([[a[0], a[3], a[4]], [a[3], a[1], a[5]], [a[4], a[5], a[2]]])
+    D -= np.trace(D) / 3.0 * np.eye(3)
+    return D
+
+
+def _skew(rng):
+    w = [2.0 * rng.uniform() - 1.0 for _ in range(3)]
+    return np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+
+
+def _vm(T):
+    return math.sqrt(2.0 / 3.0 * float((T * T).sum()))
+
+
+@dataclass
+class RveSpec:
+    """One RVE of a many-RVE workload: grid, grains, seed and velocity gradient."""
+    dims: tuple
+    n_grains: int
+    seed: int
+    velgrad: np.ndarray     # 3x3, 1/s
+
+
+def batch_specs(n=256, sizes=(16, 32, 48, 64), grains=(8, 125), seed0=4000, rate=1e-3, skew=True):
+    """Batched-RVE workload: grid N^3 with N uniform over `sizes`, grain count
+    uniform over grains[0]..grains[1], Voronoi seed seed0 + r, and
+    L = D + W with D random symmetric traceless scaled to von Mises rate
+    `rate` and W random skew on the same scale (W = 0 with skew=False)."""
+    out = []
+    for r in range(n):
+        rng = SplitMix64(seed0 + r)
+        N = sizes[min(len(sizes) - 1, int(rng.uniform() * len(sizes)))]
+        g = grains[0] + min(grains[1] - grains[0], int(rng.uniform() * (grains[1] - grains[0] + 1)))
+        D = _traceless_sym(rng)
+        D *= rate / _vm(D)
+        L = D.copy()
+        W = _skew(rng)
+        if skew:
+            L += W * (rate / _vm(W))
+        out.append(RveSpec((N, N, N), g, seed0 + r, L))
+    return out
+
+
+def serve_specs(n_ids=4096, sizes=(16, 32, 64), grains=(8, 64), seed0=5_000_000):
+    """Served workload ids: grid, grains and Voronoi seed per id (the velocity
+    gradient comes with each request, see request_velgrad)."""
+    out = []
+    for i in range(n_ids):
+        rng = SplitMix64(seed0 + i)
+        N = sizes[min(len(sizes) - 1, int(rng.uniform() * len(sizes)))]
+        g = grains[0] + min(grains[1] - grains[0], int(rng.uniform() * (grains[1] - grains[0] + 1)))
+        out.append(RveSpec((N, N, N), g, seed0 + i, np.zeros((3, 3))))
+    return out
+
+
+def request_velgrad(rng, rate=1e-3):
+    """Random traceless velocity gradient with Frobenius norm `rate`."""
+    L = np.array([[2.0 * rng.uniform() - 1.0 for _ in range(3)] for _ in range(3)])
+    L -= np.trace(L) / 3.0 * np.eye(3)
+    return L * (rate / math.sqrt(float((L * L).sum())))
+
+
+def full_velocity_case(dims=(16, 16, 16), time_step=1.0, err=1e-5, iter_max=100, n_steps=20):
+    """A Case whose namelist imposes every velocity-gradient component (the
+    per-RVE gradient replaces VelGrad_BC when the case is loaded)."""
+    return Case(dims=dims, time_step=time_step, err=err, iter_max=iter_max, n_steps=n_steps,
+                velgrad_flag=(1, 1, 1, 1, 1, 1), velgrad=(1.0e-3, -0.5e-3, -0.5e-3, 0.0, 0.0, 0.0),
+                stress_flag=(0, 0, 0, 0, 0, 0))
+
+
+def load_specs(directory, specs, base_case=None):
+    """Write one namelist + material into `directory` and load every spec
+    with its generated mesh and its own velocity gradient."""
+    from . import load_case_voronoi
+    base_case = base_case or full_velocity_case()
+    write_case(directory, base_case, grain=np.ones(base_case.nvox, dtype=np.int32),
+               euler=np.zeros((1, 3)))
+    path = os.path.join(directory, "input")
+    cases = []
+    for sp in specs:
+        cs = load_case_voronoi(path, directory, sp.dims, sp.n_grains, sp.seed)
+        cs.set_velgrad_full(sp.velgrad)
+        cases.append(cs)
+    return cases
