@@ -408,7 +408,6 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     // distinct (phase, Euler triple)
     std::vector<uint32_t> ori(V);
     std::vector<uint8_t> phv(V);
-    std::vector<double> Rv(9 * V);
     std::map<std::tuple<int, double, double, double>, uint32_t> omap;
     std::vector<double> Cgr;   // 36 per orientation
     std::vector<double> Sloc;  // 36 per orientation
@@ -416,6 +415,9 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     double c066[6][6];
     memset(c066, 0, sizeof(c066));
     long nph1 = 0;
+    std::tuple<int, double, double, double> prev_key;
+    uint32_t prev_o = 0;
+    bool have_prev = false;
     for (long v = 0; v < V; ++v) {
         const int ph = su->phase_id[v];
         if (ph < 1 || ph > su->n_phases) return VOCE_ERR_MESH_INDEX;
@@ -424,13 +426,15 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         const voce_phase &P = su->phase[ph - 1];
         if (P.type != 0) {
             ori[v] = VOCE_GAS;
-            for (int i = 0; i < 9; ++i) Rv[i * V + v] = (i % 4 == 0) ? 1.0 : 0.0;
             continue;
         }
         const double e0 = su->euler_deg[3 * v], e1 = su->euler_deg[3 * v + 1], e2 = su->euler_deg[3 * v + 2];
         auto key = std::make_tuple(ph, e0, e1, e2);
-        auto it = omap.find(key);
         uint32_t o;
+        if (have_prev && key == prev_key) {   // runs of one grain along z
+            o = prev_o;
+        } else {
+        auto it = omap.find(key);
         if (it == omap.end()) {
             o = (uint32_t)Rtab.size();
             omap[key] = o;
@@ -449,8 +453,11 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         } else {
             o = it->second;
         }
+        prev_key = key;
+        prev_o = o;
+        have_prev = true;
+        }
         ori[v] = o;
-        for (int i = 0; i < 9; ++i) Rv[i * V + v] = Rtab[o][i];
         for (int i = 0; i < 6; ++i)
             for (int j = 0; j < 6; ++j) c066[i][j] += Cgr[o * 36 + i * 6 + j];
     }
@@ -468,11 +475,13 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     // K_c = C066^-1 + S_local, the constant part of the compliance-form
     // Newton matrix of UpdateStress (evolution.c:85-93 with C066 factored out)
     std::vector<double> otab(VOCE_OTAB * (norient > 0 ? norient : 1), 0.0);
-    for (long o = 0; o < norient; ++o)
+    for (long o = 0; o < norient; ++o) {
         for (int i = 0; i < 36; ++i) {
             otab[VOCE_OTAB * o + i] = Sloc[36 * o + i];
             otab[VOCE_OTAB * o + 36 + i] = S066[i / 6][i % 6] + Sloc[36 * o + i];
         }
+        for (int i = 0; i < 9; ++i) otab[VOCE_OTAB * o + 72 + i] = Rtab[o][i];   // initial TranMat
+    }
 
     // boundary conditions (init.c:713-745)
     const int *vf = su->velgrad_flag, *sf = su->stress_flag;
@@ -604,45 +613,66 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.pl_r8 = d.x_r8 = 1;
     const size_t cb = d.fft_f32 ? 8 : 16;
     const size_t db = sizeof(double);
-    ALLOC(d.E6, 6 * V * db);
-    ALLOC(d.S6, 6 * V * db);
-    ALLOC(d.P6, 6 * V * db);
-    ALLOC(d.EP5, 5 * V * db);
-    ALLOC(d.ED5, 5 * V * db);
-    ALLOC(d.CRSS, 12 * V * db);
-    if (two_any) ALLOC(d.CRSSB, 12 * V * db);
-    else d.CRSSB = d.CRSS;
-    ALLOC(d.R, 9 * V * db);
-    ALLOC(d.WG, 3 * V * db);
-    ALLOC(d.WF, 3 * V * db);
-    ALLOC(d.GAC, V * db);
-    ALLOC(d.GTOT, 12 * V * db);
-    ALLOC(d.ORI, V * sizeof(uint32_t));
-    ALLOC(d.PH, V);
-    ALLOC(d.SLOC, otab.size() * db);
-    ALLOC(d.phase, 2 * sizeof(PhaseConst));
-    if (opt && opt->ms_dev) {
+    d.nblk_vox = (int)((V + voce_vox_block() - 1) / voce_vox_block());
+    const bool ws = !(opt && opt->no_workspace);
+    const bool own_ms = !(opt && opt->ms_dev);
+    // one device arena: per-voxel state, then the small tables, then (unless
+    // a server assigns them per request) the per-increment workspace
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_E6 = take(6 * V * db), o_S6 = take(6 * V * db), o_EP5 = take(5 * V * db), o_ED5 = take(5 * V * db);
+    const size_t o_CR = take(12 * V * db), o_CRB = two_any ? take(12 * V * db) : 0;
+    const size_t o_R = take(9 * V * db), o_WG = take(3 * V * db), o_WF = take(3 * V * db), o_GAC = take(V * db);
+    const size_t o_GT = take(12 * V * db), o_ORI = take(V * sizeof(uint32_t)), o_PH = take(V);
+    const size_t o_SL = take(otab.size() * db), o_PC = take(2 * sizeof(PhaseConst));
+    const size_t o_TX = take(nx * cb), o_TY = take(ny * cb), o_TZ = take(nz * cb), o_S6O = take(sig6o.size() * db);
+    const size_t o_TAB = take(sizeof(RveDev));
+    const size_t o_MS = own_ms ? take(sizeof(MacroState)) : 0, o_LOG = own_ms ? take(3 * (size_t)c->log_cap * db) : 0;
+    const size_t o_P6 = ws ? take(6 * V * db) : 0, o_SP = ws ? take(6 * d.nspec * cb) : 0;
+    const size_t o_US = ws ? take(3 * d.nspec * cb) : 0, o_PT = ws ? take((size_t)d.nblk_vox * 16 * db) : 0;
+    char *arena = nullptr;
+    ALLOC(arena, off);
+    c->arena_bytes = off;
+    d.E6 = (double *)(arena + o_E6);
+    d.S6 = (double *)(arena + o_S6);
+    d.EP5 = (double *)(arena + o_EP5);
+    d.ED5 = (double *)(arena + o_ED5);
+    d.CRSS = (double *)(arena + o_CR);
+    d.CRSSB = two_any ? (double *)(arena + o_CRB) : d.CRSS;
+    d.R = (double *)(arena + o_R);
+    d.WG = (double *)(arena + o_WG);
+    d.WF = (double *)(arena + o_WF);
+    d.GAC = (double *)(arena + o_GAC);
+    d.GTOT = (double *)(arena + o_GT);
+    d.ORI = (uint32_t *)(arena + o_ORI);
+    d.PH = (uint8_t *)(arena + o_PH);
+    d.SLOC = (double *)(arena + o_SL);
+    d.phase = (PhaseConst *)(arena + o_PC);
+    d.tw_x = arena + o_TX;
+    d.tw_y = arena + o_TY;
+    d.tw_z = arena + o_TZ;
+    double *sig6o_d = (double *)(arena + o_S6O);
+    c->dtab = (RveDev *)(arena + o_TAB);
+    if (own_ms) {
+        d.ms = (MacroState *)(arena + o_MS);
+        c->logs_dev = (double *)(arena + o_LOG);
+    } else {
         d.ms = opt->ms_dev;
         c->logs_dev = opt->logs_dev;
-    } else {
-        ALLOC(d.ms, sizeof(MacroState));
-        ALLOC(c->logs_dev, 3 * (size_t)c->log_cap * db);
     }
     ms0->logs = c->logs_dev;
     ms0->log_cap = c->log_cap;
-    d.nblk_vox = (int)((V + voce_vox_block() - 1) / voce_vox_block());
-    if (!(opt && opt->no_workspace)) {
-        ALLOC(d.spec, 6 * d.nspec * cb);
-        ALLOC(d.usum, 3 * d.nspec * cb);
-        ALLOC(d.part, (size_t)d.nblk_vox * 16 * db);
+    if (ws) {
+        d.P6 = (double *)(arena + o_P6);
+        d.spec = arena + o_SP;
+        d.usum = arena + o_US;
+        d.part = (double *)(arena + o_PT);
     }
-    ALLOC(d.tw_x, nx * cb);
-    ALLOC(d.tw_y, ny * cb);
-    ALLOC(d.tw_z, nz * cb);
-    ALLOC(c->dtab, sizeof(RveDev));
     if (opt) c->on_tab = opt->on_tab;
-    double *sig6o_d = nullptr;
-    ALLOC(sig6o_d, sig6o.size() * db);
     if (hipHostMalloc((void **)&c->h_ms, sizeof(MacroState), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_ctl, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         free(ms0);
@@ -650,38 +680,19 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         return VOCE_ERR_NOMEM;
     }
 #undef ALLOC
-    // host-side initial arrays
-    std::vector<double> crss(12 * V, 0.0), crssb;
-    if (two_any) crssb.assign(12 * V, 0.0);
-    for (long v = 0; v < V; ++v) {
-        if (ori[v] == VOCE_GAS) continue;
-        const PhaseConst &pc = pcs[phv[v] - 1];
-        for (int s = 0; s < pc.nsys; ++s) {
-            crss[s * V + v] = pc.tau0f;   // init.c:789-792
-            if (two_any) crssb[s * V + v] = pc.tau0b;
-        }
-    }
 #define H2D(dst, src, bytes) CK(hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, c->stream))
 #define ZERO(dst, bytes) CK(hipMemsetAsync((dst), 0, (bytes), c->stream))
     int e = 0;
     do {
-        H2D(d.CRSS, crss.data(), 12 * V * db);
-        if (two_any) H2D(d.CRSSB, crssb.data(), 12 * V * db);
-        H2D(d.R, Rv.data(), 9 * V * db);
+        // per voxel only the orientation index and phase travel; the initial
+        // TranMat, CRSS (init.c:789-792) and stress (InitFields) are expanded
+        // on the device from the per-orientation and per-phase tables
         H2D(d.ORI, ori.data(), V * sizeof(uint32_t));
         H2D(d.PH, phv.data(), V);
         H2D(d.SLOC, otab.data(), otab.size() * db);
         H2D(d.phase, pcs, 2 * sizeof(PhaseConst));
         H2D(d.ms, ms0, sizeof(MacroState));
         H2D(sig6o_d, sig6o.data(), sig6o.size() * db);
-        ZERO(d.E6, 6 * V * db);
-        ZERO(d.P6, 6 * V * db);
-        ZERO(d.EP5, 5 * V * db);
-        ZERO(d.ED5, 5 * V * db);
-        ZERO(d.WG, 3 * V * db);
-        ZERO(d.WF, 3 * V * db);
-        ZERO(d.GAC, V * db);
-        ZERO(d.GTOT, 12 * V * db);
         if (d.usum) ZERO(d.usum, 3 * d.nspec * cb);
         if (d.fft_f32) {
             std::vector<float> tx, ty, tz;

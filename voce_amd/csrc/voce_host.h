@@ -32,6 +32,7 @@ struct voce_ctx {
     std::vector<int> grain, phase_id;
     std::vector<double> log_e, log_s, log_v;
     std::vector<void *> allocs;
+    size_t arena_bytes;    // device bytes of the context's arena
     // kernel timing (include/voce_prof.h)
     int prof;
     std::vector<hipEvent_t> ev;    // 6 per enqueued iteration

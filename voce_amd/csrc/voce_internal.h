@@ -36,7 +36,7 @@
 #include <hip/hip_runtime.h>   // int2
 
 #define VOCE_GAS 0xffffffffu
-#define VOCE_OTAB 72    // doubles per orientation-table row
+#define VOCE_OTAB 81    // doubles per orientation-table row
 
 struct PhaseConst {
     int nsys;
@@ -91,7 +91,8 @@ struct RveDev {
     uint32_t *ORI;
     uint8_t *PH;
     double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1
-                                 // and K_c = C066^-1 + S_local (36 each, row major)
+                                 // and K_c = C066^-1 + S_local (36 each, row major),
+                                 // then the initial TranMat (9)
     PhaseConst *phase;           // [2]
     MacroState *ms;
     void *spec, *usum;           // complex T

@@ -1657,15 +1657,40 @@ __global__ __launch_bounds__(256) void k_count_active(const RveDev *__restrict__
     }
 }
 
-// initial fields: S6 from the per-orientation table (InitFields, init.c:486-534)
-__global__ void k_init_sig(const RveDev *__restrict__ rtab, const double *sig6_by_orient)
+// Initial state (InitFields, init.c:486-534; CRSS init.c:789-792): from the
+// orientation index and phase of each voxel, the TranMat and stress of its
+// orientation-table row, CRSS = tau0 on the phase's systems, zeros elsewhere.
+__global__ void k_init_state(const RveDev *__restrict__ rtab, const double *sig6_by_orient)
 {
-    const RveDev r = rtab[0];
+    const RveDev &r = rtab[0];
     const long V = r.V;
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const uint32_t ori = r.ORI[v];
-    for (int i = 0; i < 6; ++i) r.S6[i * V + v] = (ori == VOCE_GAS) ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
+    const bool gas = (ori == VOCE_GAS);
+    const PhaseConst *pc = r.phase + (r.PH[v] - 1);
+    const double *row = r.SLOC + (long)(gas ? 0 : ori) * VOCE_OTAB + 72;
+    for (int i = 0; i < 9; ++i) r.R[i * V + v] = gas ? (double)(i % 4 == 0) : row[i];
+    for (int i = 0; i < 6; ++i) {
+        r.S6[i * V + v] = gas ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
+        r.E6[i * V + v] = 0.0;
+        if (r.P6) r.P6[i * V + v] = 0.0;
+    }
+    for (int i = 0; i < 5; ++i) {
+        r.EP5[i * V + v] = 0.0;
+        r.ED5[i * V + v] = 0.0;
+    }
+    for (int i = 0; i < 3; ++i) {
+        r.WG[i * V + v] = 0.0;
+        r.WF[i * V + v] = 0.0;
+    }
+    r.GAC[v] = 0.0;
+    const int nsys = gas ? 0 : pc->nsys;
+    for (int s = 0; s < 12; ++s) {
+        r.GTOT[s * V + v] = 0.0;
+        r.CRSS[s * V + v] = (s < nsys) ? pc->tau0f : 0.0;
+        if (r.CRSSB != r.CRSS) r.CRSSB[s * V + v] = (s < nsys) ? pc->tau0b : 0.0;
+    }
 }
 
 #ifndef VOCE_NT_VOX
@@ -1876,6 +1901,6 @@ extern "C" int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, 
 extern "C" int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream)
 {
     const long nb = (V + 255) / 256;
-    hipLaunchKernelGGL(k_init_sig, dim3(nb), dim3(256), 0, (hipStream_t)stream, tab1, sig6_by_orient);
+    hipLaunchKernelGGL(k_init_state, dim3(nb), dim3(256), 0, (hipStream_t)stream, tab1, sig6_by_orient);
     return (int)hipGetLastError();
 }
