@@ -68,6 +68,56 @@ void voce_batch_destroy(voce_batch *b);
  * evolution.c:460 with the new L) and uses skew(L) as the applied spin. */
 int voce_set_velgrad(voce_ctx *ctx, const double L[9]);
 
+/* ---- request server (FE^2-style) ------------------------------------------ */
+
+/* Strain-increment requests against resident RVEs.  Each registered id keeps
+ * its state (stress, strain, plastic strain, CRSS, orientation, ...) on the
+ * device between requests.  A request is admitted when its id is idle, an
+ * in-flight slot is free and its per-increment workspace (spectra, partial
+ * sums, pre-update stress) fits in the workspace pool; admitted increments
+ * iterate together in the same kernels (block maps over the in-flight set),
+ * new ones join between iteration chunks and finished ones leave.  An
+ * audited request also runs, from a copy of the same state, in FP64 and
+ * reports the macro-stress difference; the id's own state is not affected. */
+typedef struct voce_server voce_server;
+
+typedef struct voce_server_opts {
+    int precision;            /* VOCE_MIXED or VOCE_FP64 for served increments */
+    int max_rves;             /* ids that can be registered (table capacity) */
+    int max_inflight;         /* increments iterating at once (0: memory-bound only) */
+    size_t workspace_bytes;   /* workspace pool (0: a quarter of the free device memory) */
+    int tune;                 /* autotune each grid size when first registered */
+    int chunk;                /* FFT iterations launched between polls (0: 2) */
+} voce_server_opts;
+
+typedef struct voce_result {
+    long tag;                 /* as submitted */
+    int id;
+    int status;               /* 0 or an error code */
+    int audited;              /* 1: the increment was also run in FP64 */
+    voce_incr_report rep;     /* macro results (err_e/err_s/svm_it are NULL) */
+    double audit_sig_rel;     /* audited: max|sig_avg - sig_avg_fp64| / max|sig_avg_fp64| */
+    double t_submit, t_admit, t_done;   /* CLOCK_MONOTONIC seconds; t_done: result on the host */
+} voce_result;
+
+int voce_server_create(const voce_server_opts *opts, voce_server **out);
+/* Register id (0 <= id < max_rves) with its RVE: builds its resident state. */
+int voce_server_add_rve(voce_server *s, int id, const voce_setup *setup);
+/* Queue one increment of id under velocity gradient L (row-major 3x3, 1/s). */
+int voce_server_submit(voce_server *s, int id, const double L[9], int audit, long tag);
+/* One scheduling round: admit, iterate one chunk, retire finished
+ * increments.  *inflight (may be NULL) receives the number of increments
+ * still in flight.  Returns 0 or an error code. */
+int voce_server_step(voce_server *s, int *inflight);
+/* Move up to max finished results (completion order) into out; returns the count. */
+int voce_server_poll(voce_server *s, voce_result *out, int max);
+/* out[0..9]: registered ids, resident bytes, workspace pool bytes, pool bytes
+ * in use, in flight, queued, completed, audits, steps, FFT iterations launched */
+int voce_server_info(const voce_server *s, long out[10]);
+/* The id's context (fields of its resident state; owned by the server). */
+voce_ctx *voce_server_rve(voce_server *s, int id);
+void voce_server_destroy(voce_server *s);
+
 /* ---- synthetic microstructures ------------------------------------------ */
 
 /* Periodic Voronoi grains on an nx*ny*nz grid (x slowest): seeds uniform in

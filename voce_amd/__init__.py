@@ -91,7 +91,9 @@ EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_fi
            # include/voce_batch.h
            "voce_autotune", "voce_apply_tuning", "voce_batch_create", "voce_batch_increment",
            "voce_batch_size", "voce_batch_member", "voce_batch_info", "voce_batch_destroy",
-           "voce_set_velgrad", "voce_voronoi", "voce_load_case_voronoi"]
+           "voce_set_velgrad", "voce_voronoi", "voce_load_case_voronoi",
+           "voce_server_create", "voce_server_add_rve", "voce_server_submit", "voce_server_step",
+           "voce_server_poll", "voce_server_info", "voce_server_rve", "voce_server_destroy"]
 
 PROF_KERNELS = ["plane_fwd", "xgreen", "plane_inv", "nr", "fin_iter"]
 
@@ -138,8 +140,29 @@ def load_library(path: str = LIB_PATH):
     lib.voce_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
     lib.voce_batch_destroy.argtypes = [C.c_void_p]
     lib.voce_batch_destroy.restype = None
+    lib.voce_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]
+    lib.voce_server_add_rve.argtypes = [C.c_void_p, C.c_int, C.POINTER(Setup)]
+    lib.voce_server_submit.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_long]
+    lib.voce_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    lib.voce_server_poll.argtypes = [C.c_void_p, C.POINTER(Result), C.c_int]
+    lib.voce_server_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+    lib.voce_server_rve.argtypes = [C.c_void_p, C.c_int]
+    lib.voce_server_rve.restype = C.c_void_p
+    lib.voce_server_destroy.argtypes = [C.c_void_p]
+    lib.voce_server_destroy.restype = None
     _lib = lib
     return lib
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_rves", C.c_int), ("max_inflight", C.c_int),
+                ("workspace_bytes", C.c_size_t), ("tune", C.c_int), ("chunk", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_long), ("id", C.c_int), ("status", C.c_int), ("audited", C.c_int),
+                ("rep", Report), ("audit_sig_rel", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
 class TuneChoice(C.Structure):
@@ -230,9 +253,10 @@ def report_dict(r: Report) -> dict:
     d = {k: getattr(r, k) for k in ("istep", "iter", "iter_total", "e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1")}
     for k in ("sig_avg", "sig_avg1", "disgrad_avg", "velgrad_avg", "eps_avg", "edot_avg"):
         d[k] = np.array(getattr(r, k)[:]).reshape(3, 3)
-    d["err_e"] = np.array(r.err_e[:n]) if n else np.zeros(0)
-    d["err_s"] = np.array(r.err_s[:n]) if n else np.zeros(0)
-    d["svm_it"] = np.array(r.svm_it[:n]) if n else np.zeros(0)
+    # per-iteration logs (NULL in server results)
+    d["err_e"] = np.array(r.err_e[:n]) if n and r.err_e else np.zeros(0)
+    d["err_s"] = np.array(r.err_s[:n]) if n and r.err_s else np.zeros(0)
+    d["svm_it"] = np.array(r.svm_it[:n]) if n and r.svm_it else np.zeros(0)
     return d
 
 
@@ -368,6 +392,88 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             load_library().voce_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """FE^2-style request server over resident RVEs (include/voce_batch.h)."""
+
+    INFO_KEYS = ("registered", "resident_bytes", "pool_bytes", "pool_used", "inflight", "queued", "completed",
+                 "audits", "steps", "iterations_launched")
+
+    def __init__(self, max_rves, precision="mixed", max_inflight=0, workspace_bytes=0, tune=True, chunk=2):
+        lib = load_library()
+        o = ServerOpts(MIXED if precision in ("mixed", MIXED) else FP64, int(max_rves), int(max_inflight),
+                       int(workspace_bytes), int(bool(tune)), int(chunk))
+        h = C.c_void_p()
+        rc = lib.voce_server_create(C.byref(o), C.byref(h))
+        if rc:
+            raise VoceError(rc, "voce_server_create")
+        self._h = h
+        self._keep = {}
+        self._nvox = {}
+
+    def add_rve(self, rid, case):
+        rc = load_library().voce_server_add_rve(self._h, int(rid), C.byref(case.setup))
+        if rc:
+            raise VoceError(rc, "voce_server_add_rve(%d)" % rid)
+        self._nvox[int(rid)] = case.nvox
+
+    def submit(self, rid, L, audit=False, tag=0):
+        a = (C.c_double * 9)(*[float(x) for x in np.asarray(L, dtype=np.float64).ravel()])
+        rc = load_library().voce_server_submit(self._h, int(rid), a, int(bool(audit)), int(tag))
+        if rc:
+            raise VoceError(rc, "voce_server_submit")
+
+    def step(self) -> int:
+        n = C.c_int(0)
+        rc = load_library().voce_server_step(self._h, C.byref(n))
+        if rc:
+            raise VoceError(rc, "voce_server_step")
+        return n.value
+
+    def poll(self, max_n=4096):
+        buf = (Result * max_n)()
+        n = load_library().voce_server_poll(self._h, buf, max_n)
+        out = []
+        for r in buf[:n]:
+            d = report_dict(r.rep)
+            d.update(tag=r.tag, id=r.id, status=r.status, audited=r.audited, audit_sig_rel=r.audit_sig_rel,
+                     t_submit=r.t_submit, t_admit=r.t_admit, t_done=r.t_done)
+            out.append(d)
+        return out
+
+    def drain(self):
+        """Step until nothing is queued or in flight; return all results."""
+        out = []
+        while True:
+            n = self.step()
+            out += self.poll()
+            info = self.info()
+            if n == 0 and info["queued"] == 0:
+                break
+        return out + self.poll()
+
+    def info(self) -> dict:
+        out = (C.c_long * 10)()
+        load_library().voce_server_info(self._h, out)
+        return dict(zip(self.INFO_KEYS, list(out)))
+
+    def rve(self, rid) -> Solver:
+        hm = load_library().voce_server_rve(self._h, int(rid))
+        if not hm:
+            raise KeyError(rid)
+        return _Member(hm, self._nvox[int(rid)], self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().voce_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -754,6 +754,7 @@ extern "C" int voce_set_diagnostics(voce_ctx *c, int on)
 extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
 {
     if (!c) return VOCE_ERR_ARG;
+    if (!c->d.spec) return VOCE_ERR_ARG;   // a server's resident RVE: increments go through the server
     RveDev &d = c->d;
     int e = voce_launch_begin_increment(&c->L, c->stream);
     if (e) return VOCE_ERR_DEVICE;

@@ -132,6 +132,7 @@ int voce_launch_iteration(const LaunchSet *L, void *stream, void *events /* hipE
 int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew);
 int voce_launch_fft_probe(const LaunchSet *L, void *stream, int which);
 int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, void *stream);
+int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_dev, void *stream);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 int voce_kernels_prepare(void);

@@ -1892,6 +1892,23 @@ extern "C" int voce_launch_closeout(const LaunchSet *L, void *stream, int need_s
     return (int)hipGetLastError();
 }
 
+// Server poll: flags[2i] = active, flags[2i+1] = error of member idx[i].
+__global__ __launch_bounds__(256) void k_poll(const RveDev *__restrict__ tab, const int *__restrict__ idx, int n, int *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MacroState *ms = tab[idx[i]].ms;
+    flags[2 * i] = ms->active;
+    flags[2 * i + 1] = ms->error;
+}
+
+extern "C" int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_dev, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_poll, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, tab, idx_dev, n, flags_dev);
+    return (int)hipGetLastError();
+}
+
 extern "C" int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, void *stream)
 {
     hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, (hipStream_t)stream, tab, n, out_dev);
