@@ -30,6 +30,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -65,7 +67,7 @@ def make_case(d, steps):
     return os.path.join(d, "input"), c
 
 
-def run_gpu(path, base, precision, steps, warmup, prof=True):
+def run_gpu(path, base, precision, steps, warmup, prof=True, tune=True):
     import voce_amd
     cs = voce_amd.load_case(path, base)
     if warmup > 0:
@@ -75,6 +77,9 @@ def run_gpu(path, base, precision, steps, warmup, prof=True):
         w.close()
         log(precision, "warmup", warmup, "increments done")
     s = voce_amd.Solver.from_case(cs, precision)
+    tuned = s.autotune() if tune else None
+    if tuned:
+        log(precision, "tuned FFT launch choice", tuned)
     if prof:
         s.prof_enable(True)
     reps = []
@@ -87,6 +92,7 @@ def run_gpu(path, base, precision, steps, warmup, prof=True):
     kp = s.prof_read() if prof else {}
     V = s.nvox
     s.close()
+    run_gpu.tuned = tuned
     return reps, wall, kp, V
 
 
@@ -109,6 +115,127 @@ def cpu_baseline(path, base, sample_iters, threads):
     return V * r["iter"] / dt, dt, r["iter"]
 
 
+def run_batched(d, precision, n_rve, steps, log_every=5):
+    """Config-4 style workload: n_rve RVEs of uneven size, each with its own
+    velocity gradient, advanced `steps` increments together."""
+    import voce_amd
+    from voce_amd import inputs
+    specs = inputs.batch_specs(n=n_rve)
+    t0 = time.perf_counter()
+    cases = inputs.load_specs(os.path.join(d, "batch_%s" % precision), specs,
+                              inputs.full_velocity_case(time_step=1.0, err=1e-5, iter_max=100))
+    b = voce_amd.Batch(cases, precision, tune=True)
+    t_setup = time.perf_counter() - t0
+    for c in cases:
+        c.free()
+    V = [sp.dims[0] * sp.dims[1] * sp.dims[2] for sp in specs]
+    log("batched %s: %d RVEs, %d voxels, setup %.1f s" % (precision, n_rve, sum(V), t_setup))
+    iters = 0
+    reps_all = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        reps = b.increment()
+        reps_all.append(reps)
+        iters += sum(v * r["iter"] for v, r in zip(V, reps))
+        if (k + 1) % log_every == 0:
+            log("batched %s increment %d, max iters %d, elapsed %.2f s" % (precision, k + 1,
+                                                                           max(r["iter"] for r in reps),
+                                                                           time.perf_counter() - t0))
+    wall = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    return {"rve_increments_per_s": round(n_rve * steps / wall, 2), "voxel_iterations_per_s": round(iters / wall, 1),
+            "wall_s": round(wall, 3), "setup_s": round(t_setup, 2), "rves": n_rve, "increments": steps,
+            "voxels": sum(V), "mean_iterations": round(float(np.mean([r["iter"] for rr in reps_all for r in rr])), 2),
+            "graph_replays": info["graph_replays"], "device_bytes": info["device_bytes"]}, reps_all
+
+
+def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5):
+    """Config-5 style stream: n_ids resident RVEs, Poisson arrivals of
+    one-increment requests (random traceless L, |L| = 1e-3/s, dt 1 s),
+    mixed precision with an FP64 audit of audit_pct % of the requests."""
+    import voce_amd
+    from voce_amd import inputs
+    specs = inputs.serve_specs(n_ids=n_ids)
+    base = inputs.full_velocity_case(time_step=1.0, err=1e-5, iter_max=100)
+    t0 = time.perf_counter()
+    srv = voce_amd.Server(n_ids, "mixed", tune=True)
+    inputs.write_case(os.path.join(d, "serve"), base, grain=np.ones(base.nvox, dtype=np.int32),
+                      euler=np.zeros((1, 3)))
+    path = os.path.join(d, "serve", "input")
+    V = {}
+    for i, sp in enumerate(specs):
+        cs = voce_amd.load_case_voronoi(path, os.path.join(d, "serve"), sp.dims, sp.n_grains, sp.seed)
+        cs.set_velgrad_full(np.zeros((3, 3)))
+        srv.add_rve(i, cs)
+        cs.free()
+        V[i] = sp.dims[0] * sp.dims[1] * sp.dims[2]
+        if (i + 1) % 256 == 0:
+            log("served: registered %d ids (%.1f s)" % (i + 1, time.perf_counter() - t0))
+    t_setup = time.perf_counter() - t0
+    info0 = srv.info()
+    log("served: %d ids resident, %.1f GB, setup %.1f s" % (n_ids, info0["resident_bytes"] / 1e9, t_setup))
+    rng = inputs.SplitMix64(seed)
+
+    def stream(rate, n, tag0):
+        """n requests; rate None = all at t = 0 (capacity)."""
+        arr, reqs = [], []
+        t = 0.0
+        for i in range(n):
+            if rate:
+                t += -np.log(1.0 - rng.uniform()) / rate
+            arr.append(t)
+            rid = min(n_ids - 1, int(rng.uniform() * n_ids))
+            aud = (((tag0 + i) * 0x9E3779B1) % 10007) < audit_pct / 100.0 * 10007
+            reqs.append((rid, inputs.request_velgrad(rng), aud))
+        res = {}
+        t0 = time.perf_counter()
+        i = 0
+        last = t0
+        while len(res) < n:
+            now = time.perf_counter()
+            while i < n and t0 + arr[i] <= now:
+                rid, L, aud = reqs[i]
+                srv.submit(rid, L, aud, tag0 + i)
+                i += 1
+            inflight = srv.step()
+            for r in srv.poll():
+                res[r["tag"] - tag0] = r
+            if inflight == 0 and srv.info()["queued"] == 0 and i < n:
+                wait = t0 + arr[i] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+            if time.perf_counter() - last > 20:
+                last = time.perf_counter()
+                log("served: %d/%d done" % (len(res), n))
+        wall = time.perf_counter() - t0
+        lat = np.array([res[k]["t_done"] - (t0 + arr[k]) for k in range(n)]) * 1e3
+        vi = sum(V[reqs[k][0]] * res[k]["iter"] for k in range(n))
+        aud = [res[k]["audit_sig_rel"] for k in range(n) if res[k]["audited"]]
+        return {"offered_per_s": (round(rate, 2) if rate else None), "requests": n,
+                "increments_per_s": round(n / wall, 2), "voxel_iterations_per_s": round(vi / wall, 1),
+                "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                "mean_iterations": round(float(np.mean([res[k]["iter"] for k in range(n)])), 2),
+                "audits": len(aud), "audit_sig_rel_max": (max(aud) if aud else None)}
+
+    cap = stream(None, n_req, 0)
+    log("served capacity: %.1f increments/s, p50 %.1f ms p99 %.1f ms" % (cap["increments_per_s"],
+                                                                        cap["latency_ms_p50"], cap["latency_ms_p99"]))
+    sweep = []
+    for j, f in enumerate(rates):
+        r = stream(f * cap["increments_per_s"], n_req, (j + 1) * 10_000_000)
+        r["load"] = f
+        sweep.append(r)
+        log("served at %.2fx capacity: %.1f inc/s, p50 %.1f ms, p99 %.1f ms" % (f, r["increments_per_s"],
+                                                                                r["latency_ms_p50"],
+                                                                                r["latency_ms_p99"]))
+    info = srv.info()
+    srv.close()
+    return {"ids": n_ids, "resident_gb": round(info0["resident_bytes"] / 1e9, 2), "setup_s": round(t_setup, 1),
+            "capacity": cap, "sweep": sweep, "steps": info["steps"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +244,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--batch-rves", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-ids", type=int, default=1024)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the RVE lives whole on one MI355X and nothing here "
@@ -166,6 +299,7 @@ def main():
         "increment_ms_p50": None,
         "roofline": roof,
         "kernels": per_kernel,
+        "fft_launch_choice": run_gpu.tuned,
     }
 
     if not args.no_mixed:
@@ -179,6 +313,29 @@ def main():
                         "ms_per_step": round(mwall / args.steps * 1e3, 3),
                         "macro_stress_max_rel_err_vs_fp64": err}
         log("mixed: %.3f s, %.4g voxel-it/s, macro rel err %.2e" % (mwall, V * miters / mwall, err))
+
+    rve_inc_fp64 = out["rve_increments_per_s"]
+    if not args.no_batch:
+        bf, reps64 = run_batched(d, "fp64", args.batch_rves, args.batch_steps)
+        bm, repsm = run_batched(d, "mixed", args.batch_rves, args.batch_steps)
+        err = 0.0
+        for a, b in zip(repsm, reps64):
+            for x, y in zip(a, b):
+                err = max(err, float(np.abs(x["sig_avg"] - y["sig_avg"]).max() / np.abs(y["sig_avg"]).max()))
+        bm["macro_stress_max_rel_err_vs_fp64"] = err
+        out["batched"] = {"workload": "%d RVEs, N^3 with N in {16,32,48,64}, 8-125 grains, own random L = D + W at "
+                                      "1e-3/s, dt 1 s, %d increments, Err 1e-5, IterMax 100" % (args.batch_rves,
+                                                                                                  args.batch_steps),
+                          "fp64": bf, "mixed": bm,
+                          "mixed_rve_inc_over_plain_fp64_single": round(bm["rve_increments_per_s"] / rve_inc_fp64, 2)}
+    if not args.no_serve:
+        sv = run_served(d, args.serve_ids, args.serve_requests, (0.25, 0.5, 1.0, 2.0))
+        sv["workload"] = ("%d resident ids, N^3 with N in {16,32,64}, 8-64 grains; Poisson arrivals of one-increment "
+                          "requests, random traceless L (|L| = 1e-3/s), dt 1 s; mixed precision, FP64 audit of 1%%"
+                          % args.serve_ids)
+        sv["capacity_over_plain_fp64_single"] = round(sv["capacity"]["increments_per_s"] / rve_inc_fp64, 2)
+        out["served"] = sv
+        out["increment_ms_p50"] = sv["sweep"][1]["latency_ms_p50"] if len(sv["sweep"]) > 1 else None
 
     if not args.no_cpu:
         try:

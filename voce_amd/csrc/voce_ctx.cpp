@@ -546,6 +546,18 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     }
     memcpy(M.S0, S0, sizeof(M.S0));
     memcpy(M.C0, C0, sizeof(M.C0));
+    {
+        // K_ik(n) = sum_jl C0_ijkl n_j n_l over m = (n0^2, n1^2, n2^2, n1n2, n0n2, n0n1)
+        const int qi[6] = {0, 1, 2, 1, 0, 0}, qk[6] = {0, 1, 2, 2, 2, 1};
+        const int pj[6] = {0, 1, 2, 1, 0, 0}, pl[6] = {0, 1, 2, 2, 2, 1};
+        for (int q = 0; q < 6; ++q)
+            for (int p = 0; p < 6; ++p) {
+                const int i = qi[q], k = qk[q], j = pj[p], l = pl[p];
+                const double g = (j == l) ? C0[i][j][k][j] : C0[i][j][k][l] + C0[i][l][k][j];
+                M.G36[6 * q + p] = g;
+                M.G36f[6 * q + p] = (float)g;
+            }
+    }
     for (int i = 0; i < 9; ++i) {
         M.Udot[i] = Udot[i / 3][i % 3];
         M.Udot_a[i] = Ua[i / 3][i % 3];
@@ -601,8 +613,8 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     memset(&d, 0, sizeof(d));
     d.nx = nx; d.ny = ny; d.nz = nz; d.nzh = nz / 2 + 1;
     d.V = V;
-    d.nspec = (long)nx * ny * d.nzh;
     d.fft_f32 = (su->precision == VOCE_MIXED);
+    voce_spec_layout(&d);
     // one crystalline phase, numbered 1: its constants are uniform across a wave
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;

@@ -20,16 +20,22 @@
 //                      Green-operator part until the close-out completes it)
 //   GAC      V f64     accumulated shear gamacum;  GTOT 12 x V gammatotal
 //   ORI      V u32     index of the voxel's initial orientation in the
-//                      per-orientation table (S_local, A^-1, A^-1 C066 with
-//                      A = I + C066 S_local; 0xffffffff = gas)
+//                      per-orientation table (S_local, K_c = C066^-1 +
+//                      S_local, initial TranMat; 0xffffffff = gas)
 //   PH       V u8      phase id (1-based)
 //
 // FFT work buffers (half spectrum along z, NZH = Nz/2 + 1, complex of the
-// FFT type T: double2 in FP64 mode, float2 in MIXED mode):
-//   SPEC  6 x Nx x Ny x NZH   index ((c*Nx + kx)*Ny + ky)*NZH + kz
-//   USUM  3 x Nx x Ny x NZH   sum over the increment's iterations of the
-//                             skew part of the Green update, components
-//                             (2,1), (0,2), (1,0) (lattice-spin recovery)
+// FFT type T: double2 in FP64 mode, float2 in MIXED mode).  A (ky, kz)
+// pair is a column col = ky*NZH + kz; columns are grouped in blocks of
+// TB = 128 bytes (8 double2 or 16 float2), NCB = ceil(Ny*NZH / TB):
+//   SPEC  6 x NSPEC   component c, plane kx, column col at
+//                     c*NSPEC + ((col/TB)*Nx + kx)*TB + col%TB,
+//                     NSPEC = NCB*TB*Nx (whole cache lines for the plane
+//                     kernels, which walk columns, and for the x pass,
+//                     which walks kx)
+//   USUM  3 x NSPEC   same layout: sum over the increment's iterations of
+//                     the skew part of the Green update, components
+//                     (2,1), (0,2), (1,0) (lattice-spin recovery)
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -59,6 +65,8 @@ struct MacroState {
     double S066[36];        // C066^-1 (B basis)
     double S0[81];
     double C0[81];
+    double G36[36];         // K(n) = C0_ijkl n_j n_l as a quadratic form (Green operator)
+    float G36f[36];
     double Udot[9], Udot_a[9], Scauchy[9];
     double udot_sym6[6];    // B basis of sym(Udot)
     int velgrad_flag[6], stress_flag[6];
@@ -85,7 +93,7 @@ struct MacroState {
 // Device pointers of one RVE.
 struct RveDev {
     int nx, ny, nz, nzh;
-    long V, nspec;               // voxels, complex points of one half-spectrum component
+    long V, nspec;               // voxels, complex elements of one spectrum component (padded)
     double *E6, *S6, *P6, *EP5, *ED5, *CRSS, *CRSSB, *R, *WG, *WF, *GAC, *GTOT;
     double *GAMDOT, *TAUX, *SVM, *EVM, *WORKP, *RHOR, *RHOC;   // diagnostics (may be null)
     uint32_t *ORI;
@@ -107,6 +115,7 @@ struct RveDev {
     // ((y,kz) columns per block), radix-8 stages allowed in the plane and
     // x passes
     int plv, xv, tk, pl_r8, x_r8;
+    int ncb;                     // column blocks of the spectrum layout (see voce_spec_layout)
 };
 
 // What one launch sequence covers: a device table of RVE descriptors and,
@@ -140,6 +149,7 @@ int voce_xgreen_tile(int nx, int xv);
 int voce_vox_block(void);   // threads per block of the per-voxel kernels
 size_t voce_plane_lds(const RveDev *r, int plv);            // 0: does not fit
 size_t voce_xgreen_lds(const RveDev *r, int xv, int tk);    // 0: does not fit
+void voce_spec_layout(RveDev *r);   // sets ncb and nspec from the grid and precision
 #ifdef __cplusplus
 }
 #endif

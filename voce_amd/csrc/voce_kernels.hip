@@ -38,6 +38,13 @@ constexpr double RSQ3 = 0.57735026918962584;
 constexpr double RSQ6 = 0.40824829046386304;
 constexpr double TWO_PI_REF = 6.28318530718;    // evp.h:58
 
+// Read-only views in the constant address space: the compiler may then use
+// scalar loads for wave-uniform addresses (material constants, C0) even in
+// kernels that store to other global memory.  Only for data that no kernel
+// writes while the view is in use.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+
 template <typename T> struct cplx_of;
 template <> struct cplx_of<double> { using t = double2; };
 template <> struct cplx_of<float> { using t = float2; };
@@ -292,8 +299,10 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         __syncthreads();
     }
     fft_lines<C, NT, MAXEL, false, true>(s, nzh, ny, 1, nzh, twy, r.pl_r8);
-    C *dst = (C *)r.spec + ((long)c * nx + x) * ny * nzh;
-    for (int e = tid; e < ny * nzh; e += NT) dst[e] = s[e];
+    // column-blocked half spectrum: runs of TB columns (128 bytes) per x
+    constexpr int TB = 128 / (int)sizeof(C);
+    C *dst = (C *)r.spec + (long)c * r.nspec;
+    for (int e = tid; e < ny * nzh; e += NT) dst[((long)(e / TB) * nx + x) * TB + (e % TB)] = s[e];
 }
 
 // ------------------------------------------------------------------
@@ -317,8 +326,9 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     const C *gtz = (const C *)r.tw_z;
     for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
     for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
-    const C *srcp = (const C *)r.spec + ((long)c * nx + x) * ny * nzh;
-    for (int e = tid; e < ny * nzh; e += NT) s[e] = srcp[e];
+    constexpr int TB = 128 / (int)sizeof(C);
+    const C *srcp = (const C *)r.spec + (long)c * r.nspec;
+    for (int e = tid; e < ny * nzh; e += NT) s[e] = srcp[((long)(e / TB) * nx + x) * TB + (e % TB)];
     __syncthreads();
     fft_lines<C, NT, MAXEL, true, true>(s, nzh, ny, 1, nzh, twy, r.pl_r8);
     // pack the half spectra of rows 2p (a) and 2p+1 (b) into a full complex
@@ -402,21 +412,27 @@ __device__ __forceinline__ void kvec(int kx, int ky, int kz, int nx, int ny, int
 // u = -omega(n) (sigma_hat n) / V with omega = (C0_ijkl n_j n_l)^-1 by
 // cofactors; d = B-basis components of sym(u (x) n); w = skew(u (x) n)
 // components (2,1), (0,2), (1,0).
+// Coefficients of K(n) = C0_ijkl n_j n_l as a quadratic form: K_q =
+// sum_p G[6q + p] m_p for the 6 independent entries q = (00, 11, 22, 12, 02,
+// 01) and m = (n0^2, n1^2, n2^2, n1 n2, n0 n2, n0 n1) (host-computed, in the
+// FFT type of the kernel).
+template <typename T> __device__ __forceinline__ cptr<T> green_coef(const MacroState *ms);
+template <> __device__ __forceinline__ cptr<double> green_coef<double>(const MacroState *ms) { return as_const(ms->G36); }
+template <> __device__ __forceinline__ cptr<float> green_coef<float>(const MacroState *ms) { return as_const(ms->G36f); }
+
 template <typename T, typename C>
-__device__ __forceinline__ void green_apply(const double *C0, const T n[3], const C a[6], T invV, C d[6], C w[3])
+__device__ __forceinline__ void green_apply(const cptr<T> G, const T n[3], const C a[6], T invV, C d[6], C w[3])
 {
-    T K[3][3];
+    const T m[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[1] * n[2], n[0] * n[2], n[0] * n[1]};
+    T Kq[6];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int q = 0; q < 6; ++q) {
+        T acc = 0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            T acc = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) acc += (T)C0[((i * 3 + j) * 3 + k) * 3 + l] * n[j] * n[l];
-            K[i][k] = acc;
-        }
+        for (int p = 0; p < 6; ++p) acc += G[6 * q + p] * m[p];
+        Kq[q] = acc;
+    }
+    const T K[3][3] = {{Kq[0], Kq[5], Kq[4]}, {Kq[5], Kq[1], Kq[3]}, {Kq[4], Kq[3], Kq[2]}};
     const T det = K[0][0] * (K[1][1] * K[2][2] - K[1][2] * K[2][1]) - K[0][1] * (K[1][0] * K[2][2] - K[1][2] * K[2][0]) +
                   K[0][2] * (K[1][0] * K[2][1] - K[2][0] * K[1][1]);
     const T id = (T)1 / det;
@@ -478,38 +494,54 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     const MacroState *ms = r.ms;
     if (mode == 0 ? !iter_active(ms) : !rve_need_skew(ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    C *s = (C *)smem;
+    // Spectra and the skew sums are stored column-blocked: component cc,
+    // plane x, column col = ky * nzh + kz at ((cc * ncb + col / TB) * nx + x)
+    // * TB + col % TB, TB columns = 128 bytes, so a block's tile is read and
+    // written in whole cache lines.  In LDS every line (one component, one
+    // column, all x) is padded to nx + 1 elements against bank conflicts.
+    constexpr int TB = 128 / (int)sizeof(C);
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
-    const int ncolt = ny * nzh;
-    C *twx = s + 6 * tk * nx;
+    const int ncolt = ny * nzh, ncb = r.ncb, NXP = nx + 1;
+    const int ncomp = (mode == 0) ? 6 : 3;
+    C *s = (C *)smem;
+    C *u = s + 6 * tk * NXP;                        // skew sums (mode 0)
+    C *twx = u + (mode == 0 ? 3 * tk * NXP : 0);
     const int tid = threadIdx.x;
     const C *gtx = (const C *)r.tw_x;
     for (int i = tid; i < nx; i += NT) twx[i] = gtx[i];
     const int col0 = lb * tk;
     const int ncol = min(tk, ncolt - col0);
-    const int ncomp = (mode == 0) ? 6 : 3;
+    const bool first = (ms->iter == 0);
+    auto gofs = [&](int cc, int x, int col) -> long {
+        return ((long)(cc * ncb + col / TB) * nx + x) * TB + (col % TB);
+    };
     const C *src = (const C *)((mode == 0) ? r.spec : r.usum);
-    const long cstride = (long)nx * ncolt;
+    C *usum = (C *)r.usum;
     for (int e = tid; e < ncomp * nx * ncol; e += NT) {
         const int cc = e / (nx * ncol);
         const int rem = e - cc * nx * ncol;
         const int x = rem / ncol, t = rem - x * ncol;
-        s[(cc * tk + t) * nx + x] = src[cc * cstride + (long)x * ncolt + col0 + t];
+        s[(cc * tk + t) * NXP + x] = src[gofs(cc, x, col0 + t)];
+    }
+    if (mode == 0 && !first) {
+        for (int e = tid; e < 3 * nx * ncol; e += NT) {
+            const int cc = e / (nx * ncol);
+            const int rem = e - cc * nx * ncol;
+            const int x = rem / ncol, t = rem - x * ncol;
+            u[(cc * tk + t) * NXP + x] = usum[gofs(cc, x, col0 + t)];
+        }
     }
     __syncthreads();
-    if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, nx, 1, twx, r.x_r8);
-    const bool first = (ms->iter == 0);
-    C *usum = (C *)r.usum;
+    if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, NXP, 1, twx, r.x_r8);
     const T invV = (T)(1.0 / (double)r.V);
-    for (int e = tid; e < nx * ncol; e += NT) {
-        const int t = e / nx, x = e - t * nx;
-        const int col = col0 + t;
-        const int ky = col / nzh, kz = col - ky * nzh;
-        const long gidx = (long)x * ncolt + col;
-        if (mode == 0) {
+    if (mode == 0) {
+        for (int e = tid; e < nx * ncol; e += NT) {
+            const int t = e / nx, x = e - t * nx;
+            const int col = col0 + t;
+            const int ky = col / nzh, kz = col - ky * nzh;
             C a[6];
 #pragma unroll
-            for (int cc = 0; cc < 6; ++cc) a[cc] = s[(cc * tk + t) * nx + x];
+            for (int cc = 0; cc < 6; ++cc) a[cc] = s[(cc * tk + t) * NXP + x];
             C d[6], w[3];
             if (x == 0 && ky == 0 && kz == 0) {
                 // Gamma(k = 0) = 0 (init.c:586-591)
@@ -520,7 +552,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
             } else {
                 T n[3];
                 kvec<T>(x, ky, kz, nx, ny, nz, n);
-                green_apply<T, C>(ms->C0, n, a, invV, d, w);
+                green_apply<T, C>(green_coef<T>(ms), n, a, invV, d, w);
                 // The reference keeps the full spectrum and takes Re() of the
                 // inverse (evolution.c:383-391), i.e. it applies the Hermitian
                 // average of Gamma at k and at the index of -k.  With round()
@@ -533,7 +565,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
                     T m[3];
                     kvec<T>((nx - x) % nx, (ny - ky) % ny, nz - kz, nx, ny, nz, m);
                     C d2[6], w2[3];
-                    green_apply<T, C>(ms->C0, m, a, invV, d2, w2);
+                    green_apply<T, C>(green_coef<T>(ms), m, a, invV, d2, w2);
 #pragma unroll
                     for (int cc = 0; cc < 6; ++cc) d[cc] = C{(T)0.5 * (d[cc].x + d2[cc].x), (T)0.5 * (d[cc].y + d2[cc].y)};
 #pragma unroll
@@ -541,23 +573,31 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
                 }
             }
 #pragma unroll
-            for (int cc = 0; cc < 6; ++cc) s[(cc * tk + t) * nx + x] = d[cc];
+            for (int cc = 0; cc < 6; ++cc) s[(cc * tk + t) * NXP + x] = d[cc];
             // sum of the skew parts over the increment (lattice spin)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                C *up = usum + i * cstride + gidx;
+                C *up = u + (i * tk + t) * NXP + x;
                 *up = first ? w[i] : cadd(*up, w[i]);
             }
         }
     }
     __syncthreads();
-    fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, nx, 1, twx, r.x_r8);
+    fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, NXP, 1, twx, r.x_r8);
     C *dst = (C *)r.spec;
     for (int e = tid; e < ncomp * nx * ncol; e += NT) {
         const int cc = e / (nx * ncol);
         const int rem = e - cc * nx * ncol;
         const int x = rem / ncol, t = rem - x * ncol;
-        dst[cc * cstride + (long)x * ncolt + col0 + t] = s[(cc * tk + t) * nx + x];
+        dst[gofs(cc, x, col0 + t)] = s[(cc * tk + t) * NXP + x];
+    }
+    if (mode == 0) {
+        for (int e = tid; e < 3 * nx * ncol; e += NT) {
+            const int cc = e / (nx * ncol);
+            const int rem = e - cc * nx * ncol;
+            const int x = rem / ncol, t = rem - x * ncol;
+            usum[gofs(cc, x, col0 + t)] = u[(cc * tk + t) * NXP + x];
+        }
     }
 }
 
@@ -627,12 +667,6 @@ __device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5]
 #define VOCE_PRAGMA_(x) _Pragma(#x)
 #define VOCE_PRAGMA(x) VOCE_PRAGMA_(x)
 
-// Read-only views in the constant address space: the compiler may then use
-// scalar loads for wave-uniform addresses (material constants, C0) even in
-// kernels that store to other global memory.  Only for data that no kernel
-// writes while the view is in use.
-template <typename T> using cptr = const __attribute__((address_space(4))) T *;
-template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
 
 #ifndef VOCE_IPOW_MODE
 #define VOCE_IPOW_MODE 0
@@ -956,11 +990,11 @@ __device__ void finalize_iteration(MacroState *ms, const double *tot)
 // Newton-Raphson stress update, UpdateStress (evolution.c:3-168).
 //
 // Per NR iteration (evolution.c:63-114) the residual and Jacobian of
-// evolution.c:70-94 are, regrouped,
-//   R = A sigma + dt C0 edot - c,   J = A + dt C0 d(edot)/d(sigma)
-// with A = I + C0 S_local (per orientation) and c = lambda + C0 (eps - eps_p);
-// the step J^-1 R is taken as M^-1 A^-1 R with M = I + dt A^-1 C0 D (a 5x5
-// solve, see below), A^-1 and A^-1 C0 coming from the orientation table.
+// evolution.c:70-94 share the left factor C0 = C066:
+//   R = C0 g,  g = K_c sigma + w + dt edot,   J = C0 K,  K = K_c + dt D
+// with K_c = C0^-1 + S_local (per orientation, from the orientation table)
+// and w = (eps_p - eps) - C0^-1 lambda, so the step J^-1 R = K^-1 g is one
+// LDL^T solve of a symmetric positive definite 6x6 (see ldlt_solve6).
 // StrainRate_eval's sums (constitutive.c:49-64) enter one slip system at a
 // time from the rotated slip vectors n' = R n, b' = R b: the resolved shear
 // n' . sigma b', the Schmid vector P_s = B(sym(n' (x) b')), edot += P_s
@@ -1846,8 +1880,18 @@ extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
 {
     if (xv < 0 || xv > 2 || tk < 1 || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
-    const size_t b = (size_t)(6 * tk * r->nx + r->nx) * cb;
+    // 6 spectra + 3 skew sums, lines padded to nx + 1, then the twiddles
+    const size_t b = (size_t)(9 * tk * (r->nx + 1) + r->nx) * cb;
     return b <= LDS_MAX ? b : 0;
+}
+
+// Column-blocked spectrum layout of an RVE: ncb blocks of TB = 128 bytes of
+// columns; nspec = complex elements per component (padded).
+extern "C" void voce_spec_layout(RveDev *r)
+{
+    const int tb = r->fft_f32 ? 16 : 8;
+    r->ncb = (r->ny * r->nzh + tb - 1) / tb;
+    r->nspec = (long)r->ncb * tb * r->nx;
 }
 
 extern "C" int voce_launch_begin_increment(const LaunchSet *L, void *stream)

@@ -349,7 +349,10 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     const size_t db = sizeof(double), cb = 16;
     const int two = d0.CRSSB != d0.CRSS;
     // layout: evolving arrays, P6, spectra, sums, partials, MacroState, logs, twiddles
-    const size_t nspec = d0.nspec;
+    RveDev d = d0;   // constant tables (ORI, PH, SLOC, phase) stay aliased
+    d.fft_f32 = 0;
+    voce_spec_layout(&d);   // FP64 spectrum layout
+    const size_t nspec = d.nspec;
     size_t need = 0;
     const size_t sz_state[11] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
                                  9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db};
@@ -370,8 +373,6 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     }
     sh.src = src;
     sh.V = V;
-    RveDev d = d0;   // constant tables (ORI, PH, SLOC, phase) stay aliased
-    d.fft_f32 = 0;
     d.GAMDOT = d.TAUX = d.SVM = d.EVM = d.WORKP = d.RHOR = d.RHOC = nullptr;
     char *p = sh.arena;
     double *dst[11];
