@@ -17,12 +17,13 @@ extern "C" {
 #endif
 
 /* kernels of one FFT iteration, in launch order */
-#define VOCE_PROF_PLANE_FWD 0   /* z r2c + y FFT of the stress */
+#define VOCE_PROF_PLANE_FWD 0   /* z r2c + y FFT of the stress, and (one extra block)
+                                   the previous iteration's finaliser */
 #define VOCE_PROF_XGREEN 1      /* x FFT, Green operator, inverse x FFT */
 #define VOCE_PROF_PLANE_INV 2   /* inverse y FFT, z c2r, strain update */
 #define VOCE_PROF_NR 3          /* per-voxel Newton-Raphson and block partial sums */
-#define VOCE_PROF_FIN 4         /* iteration finaliser: partials summed in block
-                                   order, error norms, get_smacro, convergence */
+#define VOCE_PROF_FIN 4         /* reserved: 0 (the finaliser runs inside the next
+                                   forward-plane launch) */
 #define VOCE_PROF_NK 5
 
 /* Turn timing on (1) or off (0); either way the totals are reset. */

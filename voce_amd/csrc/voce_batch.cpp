@@ -64,11 +64,12 @@ static int balloc(voce_batch *b, void **p, size_t bytes)
 // block maps of one group, uploaded into one buffer
 static int build_group_maps(voce_batch *b, Group &g)
 {
-    std::vector<int2> p6, p3, x, vox, one;
+    std::vector<int2> pf, p6, p3, x, vox, one;
     bool uni = true, two = false;
     size_t pl_lds = 0, x_lds = 0;
     for (int i : g.members) {
         const RveDev &d = b->htab[i];
+        for (int k = 0; k <= d.nx * 6; ++k) pf.push_back(make_int2(i, k));   // + finaliser block
         for (int k = 0; k < d.nx * 6; ++k) p6.push_back(make_int2(i, k));
         for (int k = 0; k < d.nx * 3; ++k) p3.push_back(make_int2(i, k));
         const int nxb = (d.ny * d.nzh + d.tk - 1) / d.tk;
@@ -83,7 +84,7 @@ static int build_group_maps(voce_batch *b, Group &g)
         if (c > x_lds) x_lds = c;
         if (d.plv != b->htab[g.members[0]].plv || d.xv != b->htab[g.members[0]].xv) return VOCE_ERR_ARG;
     }
-    const size_t tot = p6.size() + p3.size() + x.size() + vox.size() + one.size();
+    const size_t tot = pf.size() + p6.size() + p3.size() + x.size() + vox.size() + one.size();
     int2 *dm = nullptr;
     int rc = balloc(b, (void **)&dm, tot * sizeof(int2));
     if (rc) return rc;
@@ -99,6 +100,7 @@ static int build_group_maps(voce_batch *b, Group &g)
         all.insert(all.end(), v.begin(), v.end());
         off += v.size();
     };
+    put(pf, &L.map_pfwd, &L.n_pfwd);
     put(p6, &L.map_plane6, &L.n_plane6);
     put(p3, &L.map_plane3, &L.n_plane3);
     put(x, &L.map_x, &L.n_x);
@@ -251,6 +253,9 @@ static int graph_for(voce_batch *b, int k, hipGraphExec_t *ex)
                 rc = VOCE_ERR_DEVICE;
                 break;
             }
+    // the chunk's last Newton step is finalised by a launch of its own
+    for (Group &gr : b->groups)
+        if (!rc && voce_launch_finalize(&gr.L, b->stream)) rc = VOCE_ERR_DEVICE;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);

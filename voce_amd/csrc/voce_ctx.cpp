@@ -255,6 +255,7 @@ int voce_sync_tab(voce_ctx *c)
     LaunchSet &L = c->L;
     memset(&L, 0, sizeof(L));
     L.tab = c->dtab;
+    L.n_pfwd = d.nx * 6 + 1;   // + the finaliser block
     L.n_plane6 = d.nx * 6;
     L.n_plane3 = d.nx * 3;
     L.n_x = (d.ny * d.nzh + d.tk - 1) / d.tk;
@@ -796,6 +797,8 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             if (e) return VOCE_ERR_DEVICE;
         }
         launched += n;
+        // the chunk's last Newton step is finalised by a launch of its own
+        if (voce_launch_finalize(&c->L, c->stream)) return VOCE_ERR_DEVICE;
         CK(hipMemcpyAsync(c->h_ctl, (char *)d.ms + ctl_off, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         CK(hipStreamSynchronize(c->stream));
         if (c->h_ctl[4]) return c->h_ctl[4];
@@ -803,7 +806,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             // only the iterations that ran (the rest returned at the flag check)
             const int ran = c->h_ctl[1] - done_before;
             for (int i = 0; i < ran && i < n; ++i)
-                for (int k = 0; k < VOCE_PROF_NK; ++k) {   // events ev[0..5] bracket the 5 kernels
+                for (int k = 0; k < VOCE_PROF_FIN; ++k) {   // events ev[0..4] bracket the 4 kernels
                     float ms = 0.f;
                     if (hipEventElapsedTime(&ms, c->ev[6 * i + k], c->ev[6 * i + k + 1]) == hipSuccess) {
                         c->prof_ms[k] += ms;

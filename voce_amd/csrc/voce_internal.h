@@ -82,7 +82,7 @@ struct MacroState {
     double creep_sym6[6];    // creep mode: uniform sym(DisGrad) at predictor
     double creep_skew[3];    // creep mode: uniform skew(DisGrad) at predictor
     int istep, iter, iter_total, active;
-    int error, pad_t, log_cap, vg_pending;
+    int error, fin_pending, log_cap, vg_pending;   // fin_pending: Newton partials not yet finalised
     // per-iteration Err_e, Err_s, s_vm of the running increment: 3 x log_cap
     double *logs;
     // velocity gradient for the next increment (taken by k_begin_increment
@@ -125,8 +125,10 @@ struct RveDev {
 // two_crss the OR of the members'.
 struct LaunchSet {
     const RveDev *tab;
-    const int2 *map_plane6, *map_plane3, *map_x, *map_vox, *map_one;
-    int n_plane6, n_plane3, n_x, n_vox, n_one;
+    // map_pfwd: the forward plane blocks plus one finaliser block per RVE
+    // (local block 6 nx), which finishes the previous iteration's Newton step
+    const int2 *map_pfwd, *map_plane6, *map_plane3, *map_x, *map_vox, *map_one;
+    int n_pfwd, n_plane6, n_plane3, n_x, n_vox, n_one;
     int plv, xv;                 // kernel variants (same for every RVE of the set)
     size_t pl_lds, x_lds;        // dynamic LDS: the largest over the set's RVEs
     int fft_f32, uniform_phase, two_crss;
@@ -141,6 +143,7 @@ int voce_launch_iteration(const LaunchSet *L, void *stream, void *events /* hipE
 int voce_launch_closeout(const LaunchSet *L, void *stream, int need_skew);
 int voce_launch_fft_probe(const LaunchSet *L, void *stream, int which);
 int voce_launch_count_active(const RveDev *tab, int n, int *out_dev, void *stream);
+int voce_launch_finalize(const LaunchSet *L, void *stream);   // pending iteration finalisers (one block per RVE)
 int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_dev, void *stream);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);

@@ -229,6 +229,8 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
     }                                                                        \
     const RveDev &r = rtab[rid_]
 
+template <int NT> __device__ __forceinline__ void finalize_block(const RveDev &r);
+
 __device__ __forceinline__ bool iter_active(const MacroState *ms)
 {
     return ms->active != 0;
@@ -250,6 +252,12 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
 {
     using C = typename cplx_of<T>::t;
     RVE_PROLOGUE(lb);
+    if (lb == 6 * r.nx) {   // the previous iteration's finaliser
+        finalize_block<NT>(r);
+        return;
+    }
+    // a finaliser may be deciding convergence beside this block: either
+    // value of the flag is fine here, the output is scratch
     if (!iter_active(r.ms)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
@@ -829,7 +837,20 @@ __device__ __forceinline__ void sum_partials(const double *part, int nb, double 
     double a[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) a[i] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += NT)
+    // rows b, b + NT, b + 2 NT, ... in that order; four rows' loads in flight at once
+    int b = threadIdx.x;
+    for (; b + 3 * NT < nb; b += 4 * NT) {
+        double q[4][NV];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) q[k][i] = __builtin_nontemporal_load(part + (long)(b + k * NT) * 16 + i);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) a[i] += q[k][i];
+    }
+    for (; b < nb; b += NT)
 #pragma unroll
         for (int i = 0; i < NV; ++i) a[i] += __builtin_nontemporal_load(part + (long)b * 16 + i);
 #pragma unroll
@@ -906,26 +927,26 @@ __device__ double svm_of(const double S[9])
 
 // Finaliser of one iteration (evolution.c:405-428 and get_smacro,
 // evolution.c:170-241), run by thread 0 on the summed partials.
-__device__ void finalize_iteration(MacroState *ms, const double *tot)
+__device__ void finalize_iteration(MacroState *ms, const MacroState *in, const double *tot)
 {
     // all reads first, then the arithmetic, then all writes: loads and
     // stores share one wait counter, so interleaving them would serialise
     // every access
-    const double wgt = ms->wgt, wgt1 = ms->wgt_over_ph1, e_vm = ms->e_vm, err_tol = ms->err_tol;
-    const int it = ms->iter, itot = ms->iter_total, itmax = ms->iter_max;
+    const double wgt = in->wgt, wgt1 = in->wgt_over_ph1, e_vm = in->e_vm, err_tol = in->err_tol;
+    const int it = in->iter, itot = in->iter_total, itmax = in->iter_max;
     int vfl[6], sfl[6];
     double Sc[9], acum[9], applied[9], S0s[6][6];
     const int IJV0[6] = {0, 1, 2, 1, 0, 0}, IJV1[6] = {0, 1, 2, 2, 2, 1};
     for (int i = 0; i < 6; ++i) {
-        vfl[i] = ms->velgrad_flag[i];
-        sfl[i] = ms->stress_flag[i];
+        vfl[i] = in->velgrad_flag[i];
+        sfl[i] = in->stress_flag[i];
         for (int k = 0; k < 6; ++k)
-            S0s[i][k] = ms->S0[((IJV0[i] * 3 + IJV1[i]) * 3 + IJV0[k]) * 3 + IJV1[k]];
+            S0s[i][k] = in->S0[((IJV0[i] * 3 + IJV1[i]) * 3 + IJV0[k]) * 3 + IJV1[k]];
     }
     for (int i = 0; i < 9; ++i) {
-        Sc[i] = ms->Scauchy[i];
-        acum[i] = ms->acum[i];
-        applied[i] = ms->applied[i];
+        Sc[i] = in->Scauchy[i];
+        acum[i] = in->acum[i];
+        applied[i] = in->applied[i];
     }
     double s6[6], s16[6], SA[9], SA1[9];
     for (int i = 0; i < 6; ++i) { s6[i] = tot[2 + i]; s16[i] = tot[8 + i]; }
@@ -969,9 +990,9 @@ __device__ void finalize_iteration(MacroState *ms, const double *tot)
     ms->s_vm1 = s_vm1;
     ms->Err_e = Err_e;
     ms->Err_s = Err_s;
-    if (it < ms->log_cap) {
-        double *lg = ms->logs;
-        const int cap = ms->log_cap;
+    if (it < in->log_cap) {
+        double *lg = in->logs;
+        const int cap = in->log_cap;
         lg[it] = Err_e;
         lg[cap + it] = Err_s;
         lg[2 * cap + it] = s_vm;
@@ -982,8 +1003,7 @@ __device__ void finalize_iteration(MacroState *ms, const double *tot)
         for (int i = 0; i < 9; ++i) ms->applied[i] = applied[i];
         for (int i = 0; i < 6; ++i) ms->shift6[i] = sh[i];
     }
-    __threadfence();
-    ms->active = more;
+    ms->active = more;   // read by the next kernels (the launch boundary orders it)
 }
 
 // ------------------------------------------------------------------
@@ -1309,23 +1329,45 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
         }
     }
     block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
+    if (lb == 0 && threadIdx.x == 0) ms->fin_pending = 1;   // read after this kernel ends
 }
 
 // Iteration finaliser: sums the Newton kernel's block partials in block
-// order and runs the error norms, get_smacro and the convergence test.  A
-// kernel of its own (one block per RVE): the kernel boundary makes the
-// partials visible without a device-scope fence in every Newton block,
-// which would write back and invalidate the L2 each time.
+// order and runs the error norms, get_smacro and the convergence test, when
+// a Newton step is pending.  It runs in one extra block of the next
+// iteration's forward plane launch (whose other blocks only read S6 and write
+// scratch spectra, so they may run beside it), or alone (k_fin_iter) after
+// the last launched iteration.  The kernel boundary before it makes the
+// Newton partials visible without a device-scope fence in every Newton
+// block, which would write back and invalidate the L2 each time.
+template <int NT>
+__device__ __forceinline__ void finalize_block(const RveDev &r)
+{
+    MacroState *ms = r.ms;
+    if (!ms->fin_pending) return;
+    // the macro state comes into LDS with all threads while the partials
+    // are summed, so the one-thread finaliser reads it at LDS latency
+    __shared__ MacroState sms;
+    static_assert(sizeof(MacroState) % 8 == 0, "MacroState is copied as doubles");
+    {
+        const double *src = (const double *)ms;
+        double *dst = (double *)&sms;
+        for (int i = threadIdx.x; i < (int)(sizeof(MacroState) / 8); i += NT) dst[i] = src[i];
+    }
+    __shared__ double tot[16];
+    sum_partials<NT, 14>(r.part, r.nblk_vox, tot);   // ends with a barrier
+    if (threadIdx.x == 0) {
+        finalize_iteration(ms, &sms, tot);
+        ms->fin_pending = 0;
+    }
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_fin_iter(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     RVE_PROLOGUE(lb);
     (void)lb;
-    MacroState *ms = r.ms;
-    if (!iter_active(ms)) return;
-    __shared__ double tot[16];
-    sum_partials<NT, 14>(r.part, r.nblk_vox, tot);
-    if (threadIdx.x == 0) finalize_iteration(ms, tot);
+    finalize_block<NT>(r);
 }
 
 // increment start (evolution.c:306-340)
@@ -1351,6 +1393,7 @@ __global__ void k_begin_increment(const RveDev *__restrict__ rtab, const int2 *_
     }
     ms->istep += 1;
     ms->iter = 0;
+    ms->fin_pending = 0;
     for (int i = 0; i < 9; ++i) {
         ms->dD[i] = 0.0;
         ms->acum[i] = 0.0;
@@ -1746,9 +1789,9 @@ template <typename T>
 void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
 {
     switch (L->plv) {
-    case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_plane6), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_plane6); break;
-    case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_plane6), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_plane6); break;
-    default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_plane6), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_plane6); break;
+    case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_pfwd), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_pfwd), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
 
@@ -1802,8 +1845,6 @@ hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
     MARK();
     launch_nr(L, st);
     MARK();
-    hipLaunchKernelGGL((k_fin_iter<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, st, L->tab, L->map_one);
-    MARK();
 #undef MARK
     return hipGetLastError();
 }
@@ -1826,7 +1867,11 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen<T, NT_X, MAXEL_X>, (const void *)k_xgreen<T, NT_XB, MAXEL_XB>,
                          (const void *)k_xgreen<T, NT_XC, MAXEL_XC>};
     for (const void *f : fns) {
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, f);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, big - (int)fa.sharedSizeBytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1868,12 +1913,15 @@ extern "C" int voce_xgreen_tile(int nx, int xv)
 // Dynamic LDS of the plane and x-pass kernels of one RVE under its tile
 // choices; 0 when a choice does not fit (tile larger than the variant's
 // MAXEL or LDS beyond 160 KB).
+// static LDS of the forward plane kernel's finaliser block, bounded above
+static const size_t PLANE_STATIC_LDS = sizeof(MacroState) + 16 * 8 + (NT_PL_L / 64) * 14 * 8 + 256;
+
 extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 {
     if (plv < 0 || plv > 2 || r->ny * r->nzh > PL_MAXEL[plv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
     const size_t b = (size_t)(r->ny * r->nzh + r->ny + r->nz) * cb;
-    return b <= LDS_MAX ? b : 0;
+    return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
 }
 
 extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
@@ -1914,6 +1962,12 @@ extern "C" int voce_launch_fft_probe(const LaunchSet *L, void *stream, int which
         else if (which == 1) launch_xgreen<double>(L, st, 0);
         else launch_plane_inv<double>(L, st, 0);
     }
+    return (int)hipGetLastError();
+}
+
+extern "C" int voce_launch_finalize(const LaunchSet *L, void *stream)
+{
+    hipLaunchKernelGGL((k_fin_iter<NT_FIN>), dim3(L->n_one), dim3(NT_FIN), 0, (hipStream_t)stream, L->tab, L->map_one);
     return (int)hipGetLastError();
 }
 

@@ -469,7 +469,7 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
         for (int r : kv.second) {
             const RveDev &d = s->htab[r];
             if (d.plv != d0.plv || d.xv != d0.xv) return VOCE_ERR_ARG;
-            need += (size_t)d.nx * 9 + (d.ny * d.nzh + d.tk - 1) / d.tk + d.nblk_vox + 1;
+            need += (size_t)d.nx * 15 + (d.ny * d.nzh + d.tk - 1) / d.tk + d.nblk_vox + 2;
         }
         if (off + need > cap) return VOCE_ERR_NOMEM;
         auto put = [&](int count_per, int kind, const int2 **mp, int *np) {
@@ -479,7 +479,8 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
             for (int r : kv.second) {
                 const RveDev &d = s->htab[r];
                 int m = 0;
-                if (kind == 0) m = d.nx * 6;
+                if (kind == 5) m = d.nx * 6 + 1;   // forward planes + finaliser block
+                else if (kind == 0) m = d.nx * 6;
                 else if (kind == 1) m = d.nx * 3;
                 else if (kind == 2) m = (d.ny * d.nzh + d.tk - 1) / d.tk;
                 else if (kind == 3) m = d.nblk_vox;
@@ -489,6 +490,7 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
             *np = n;
             off += n;
         };
+        put(0, 5, &L.map_pfwd, &L.n_pfwd);
         put(0, 0, &L.map_plane6, &L.n_plane6);
         put(0, 1, &L.map_plane3, &L.n_plane3);
         put(0, 2, &L.map_x, &L.n_x);
@@ -551,7 +553,7 @@ static size_t map_need(voce_server *s, const std::vector<int> &rows)
     size_t n = 0;
     for (int r : rows) {
         const RveDev &d = s->htab[r];
-        n += (size_t)d.nx * 9 + (d.ny * d.nzh + d.tk - 1) / d.tk + d.nblk_vox + 1;
+        n += (size_t)d.nx * 15 + (d.ny * d.nzh + d.tk - 1) / d.tk + d.nblk_vox + 2;
     }
     return n;
 }
@@ -690,6 +692,8 @@ static int server_step(voce_server *s)
     for (int it = 0; it < s->o.chunk; ++it)
         for (LaunchSet &L : s->iter_sets)
             if (voce_launch_iteration(&L, s->stream, nullptr)) return VOCE_ERR_DEVICE;
+    for (LaunchSet &L : s->iter_sets)   // the chunk's last Newton step
+        if (voce_launch_finalize(&L, s->stream)) return VOCE_ERR_DEVICE;
     s->iters += s->o.chunk;
     // 4. poll and close out the finished rows
     for (size_t i = 0; i < rows.size(); ++i) s->h_idx[i] = rows[i];
