@@ -272,8 +272,20 @@ def main():
     avg_ms = kern[dom][0]
     bpv = kernel_bytes_per_voxel(dom, nx, ny, nz, False, case.material.tau0f != case.material.tau0b)
     achieved = bpv * V / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # measured HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # (scaled by a copy-kernel calibration) committed with the profile summary
+    traffic, tsrc = None, None
+    tf = os.path.join(ROOT, "profiles", "iteration_kernels_64cube_fp64.traffic.json")
+    if os.path.exists(tf):
+        try:
+            tj = json.load(open(tf))
+            traffic = tj["bytes_per_launch"].get(dom)
+            tsrc = "profiles/iteration_kernels_64cube_fp64.md"
+        except Exception:
+            traffic = None
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": (round(traffic) if traffic else None),
+            "traffic_source": tsrc, "algorithmic_bytes": round(bpv * V),
             "bytes_per_voxel": round(bpv, 2), "avg_launch_ms": round(avg_ms, 5)}
     per_kernel = {k: {"avg_ms": round(v[0], 5), "launches": v[1], "share": round(v[2] / max(1e-12, sum(x[2] for x in kern.values())), 4)}
                   for k, v in kern.items()}
