@@ -666,9 +666,6 @@ __device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5]
 #ifndef VOCE_NR_WAVES
 #define VOCE_NR_WAVES 2      // waves per SIMD the Newton kernel is compiled for
 #endif
-#ifndef VOCE_NR_R_LDS
-#define VOCE_NR_R_LDS 1      // 1: stage the voxel's rotation in LDS; 0: re-read it
-#endif
 #ifndef VOCE_NR_SYS_UNROLL
 #define VOCE_NR_SYS_UNROLL 1
 #endif
@@ -1118,11 +1115,8 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
     if (!iter_active(ms)) return;
     // per-lane values that stay constant through the Newton loop live in LDS
     // (component-major: conflict-free), keeping the register file for the loop
-    __shared__ double lds_w[6][NT];     // A^-1 c
+    __shared__ double lds_w[6][NT];     // w = (eps_p - eps) - C066^-1 lambda
     __shared__ double lds_old[6][NT];   // stress at the last flow-rule evaluation
-#if VOCE_NR_R_LDS
-    __shared__ double lds_R[9][NT];     // TranMat of the voxel
-#endif
     __shared__ double lds_ic[TWO ? 24 : 12][NT];   // 1 / CRSS, forward (and backward)
     const long V = r.V;
     const long v = (long)lb * NT + threadIdx.x;
@@ -1145,10 +1139,6 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
             const double dt = msc->dt;
             const cptr<double> C0 = msc->C066;
             const double *tab = r.SLOC + (long)ori * VOCE_OTAB;   // S_local, A^-1, A^-1 C0
-#if VOCE_NR_R_LDS
-#pragma unroll
-            for (int i = 0; i < 9; ++i) lds_R[i][threadIdx.x] = r.R[i * V + v];
-#endif
             // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
             // it would form per evaluation)
 #pragma unroll
@@ -1191,20 +1181,14 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
 #pragma unroll
                 for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
                 double Rm[9];
-#if VOCE_NR_R_LDS
+                // TranMat is re-read (an L2 hit) each Newton step: kept out of
+                // LDS, the block fits 3 waves per SIMD
 #pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = lds_R[i][threadIdx.x];
-#else
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];   // re-read (L2) each Newton step
-#endif
-                double T[9];
-                b6_to_t(sig, T);
+                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
                 // StrainRate_eval (constitutive.c:4-72): edot and the
                 // symmetric 5x5 d(edot)/d(sigma), one slip system at a time.
                 // With n' = R n and b' = R b the sample-frame Schmid tensor is
-                // sym(n' (x) b') (update_schmid, kinematics.c:426-459) and the
-                // resolved shear is n' . sigma b'.
+                // sym(n' (x) b') (update_schmid, kinematics.c:426-459).
                 double D[15];
 #pragma unroll
                 for (int a = 0; a < 5; ++a) ed[a] = 0.0;
@@ -1232,10 +1216,20 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                         n[i] = Rm[i * 3 + 0] * bn[0] + Rm[i * 3 + 1] * bn[1] + Rm[i * 3 + 2] * bn[2];
                         bv[i] = Rm[i * 3 + 0] * bb[0] + Rm[i * 3 + 1] * bb[1] + Rm[i * 3 + 2] * bb[2];
                     }
+                    // B-basis components of sym(n' (x) b'), the sample-frame Schmid vector
+                    double P[5];
+                    {
+                        const double d0 = n[0] * bv[0], d1 = n[1] * bv[1], d2 = n[2] * bv[2];
+                        P[0] = RSQ2 * (d1 - d0);
+                        P[1] = RSQ6 * (2. * d2 - d0 - d1);
+                        P[2] = RSQ2 * (n[1] * bv[2] + n[2] * bv[1]);
+                        P[3] = RSQ2 * (n[0] * bv[2] + n[2] * bv[0]);
+                        P[4] = RSQ2 * (n[0] * bv[1] + n[1] * bv[0]);
+                    }
+                    // resolved shear: the 5-component dot of constitutive.c:26-27
                     double rss0 = 0.0;
 #pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        rss0 += n[i] * (T[i * 3 + 0] * bv[0] + T[i * 3 + 1] * bv[1] + T[i * 3 + 2] * bv[2]);
+                    for (int a = 0; a < 5; ++a) rss0 += P[a] * sig[a];
                     const double itc = (rss0 < 0) ? icb : icf;
                     const double rr = rss0 * itc;
                     double pn1, pn;
@@ -1249,16 +1243,6 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                     }
                     const double rss1 = g0 * nsr * pn1 * itc;
                     const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
-                    // B-basis components of sym(n' (x) b')
-                    double P[5];
-                    {
-                        const double d0 = n[0] * bv[0], d1 = n[1] * bv[1], d2 = n[2] * bv[2];
-                        P[0] = RSQ2 * (d1 - d0);
-                        P[1] = RSQ6 * (2. * d2 - d0 - d1);
-                        P[2] = RSQ2 * (n[1] * bv[2] + n[2] * bv[1]);
-                        P[3] = RSQ2 * (n[0] * bv[2] + n[2] * bv[0]);
-                        P[4] = RSQ2 * (n[0] * bv[1] + n[1] * bv[0]);
-                    }
 #pragma unroll
                     for (int a = 0; a < 5; ++a) ed[a] += P[a] * rss2;
                     int q = 0;
