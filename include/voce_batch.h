@@ -87,7 +87,9 @@ typedef struct voce_server_opts {
     int max_inflight;         /* increments iterating at once (0: memory-bound only) */
     size_t workspace_bytes;   /* workspace pool (0: a quarter of the free device memory) */
     int tune;                 /* autotune each grid size when first registered */
-    int chunk;                /* FFT iterations launched between polls (0: 2) */
+    int chunk;                /* most FFT iterations launched between polls (0: 16); a round
+                                 launches up to the nearest predicted end of an increment in
+                                 flight (each id's previous iteration count), at least 2 */
 } voce_server_opts;
 
 typedef struct voce_result {

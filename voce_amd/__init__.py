@@ -407,7 +407,7 @@ class Server:
     INFO_KEYS = ("registered", "resident_bytes", "pool_bytes", "pool_used", "inflight", "queued", "completed",
                  "audits", "steps", "iterations_launched")
 
-    def __init__(self, max_rves, precision="mixed", max_inflight=0, workspace_bytes=0, tune=True, chunk=2):
+    def __init__(self, max_rves, precision="mixed", max_inflight=0, workspace_bytes=0, tune=True, chunk=16):
         lib = load_library()
         o = ServerOpts(MIXED if precision in ("mixed", MIXED) else FP64, int(max_rves), int(max_inflight),
                        int(workspace_bytes), int(bool(tune)), int(chunk))

@@ -141,7 +141,7 @@ struct Inflight {
     Request req;
     double t_admit;
     int shadow;              // index into shadows, or -1
-    bool done = false;       // this row finished
+    int launched = 0;        // FFT iterations launched since admission
 };
 
 struct PendingAudit {       // a finished served row waiting for its shadow (or vice versa)
@@ -161,6 +161,7 @@ struct voce_server {
     std::vector<voce_ctx *> ctx;            // by id
     std::vector<Ws> ws;                     // by row
     std::vector<int> busy;                  // by row
+    std::vector<int> pred;                  // by id: iterations its last increment took
     Pool pool;
     char *pool_mem = nullptr;
     std::vector<Shadow> shadows;
@@ -217,7 +218,7 @@ extern "C" int voce_server_create(const voce_server_opts *opts, voce_server **ou
     if (opts->max_rves <= 0 || (opts->precision != VOCE_FP64 && opts->precision != VOCE_MIXED)) return VOCE_ERR_ARG;
     voce_server *s = new voce_server();
     s->o = *opts;
-    if (s->o.chunk <= 0) s->o.chunk = 2;
+    if (s->o.chunk <= 0) s->o.chunk = 16;
     s->max_shadow = 16;
     s->rows = s->o.max_rves + s->max_shadow;
     int rc = 0;
@@ -237,6 +238,7 @@ extern "C" int voce_server_create(const voce_server_opts *opts, voce_server **ou
     for (int i = 0; i < 16; ++i) s->h_one[i] = (i == 1) ? 1 : 0;
     if (hipMalloc((void **)&s->dtab, s->rows * sizeof(RveDev)) != hipSuccess) FAIL(VOCE_ERR_NOMEM);
     s->ctx.assign(s->o.max_rves, nullptr);
+    s->pred.assign(s->o.max_rves, 0);
     s->ws.assign(s->rows, Ws());
     s->busy.assign(s->rows, 0);
     s->shadows.resize(s->max_shadow);
@@ -300,6 +302,7 @@ extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
     int rc = voce_ctx_build(&cp, &bo, &c);
     if (rc) return rc;
     s->ctx[id] = c;
+    s->pred[id] = su->iter_max > 0 ? su->iter_max : 1;
     s->resident_bytes += (long)c->arena_bytes;
     if (s->o.tune) {
         const auto key = std::make_tuple(c->dims[0], c->dims[1], c->dims[2]);
@@ -689,12 +692,23 @@ static int server_step(voce_server *s)
         SCK(hipMemcpyAsync(s->d_maps + cap, s->h_maps + cap, used * sizeof(int2), hipMemcpyHostToDevice, s->stream));
         s->set_changed = false;
     }
-    for (int it = 0; it < s->o.chunk; ++it)
+    // chunk: up to the nearest predicted end among the rows in flight (each
+    // id's last increment), between 2 and the configured maximum; a short
+    // chunk lets finished rows leave and queued requests join sooner
+    int chunk = s->o.chunk;
+    for (const Inflight &f : s->inflight) {
+        const int rem = s->pred[f.req.id] - f.launched;
+        if (rem < chunk) chunk = rem;
+    }
+    if (chunk < 2) chunk = 2;
+    if (chunk > s->o.chunk) chunk = s->o.chunk;
+    for (int it = 0; it < chunk; ++it)
         for (LaunchSet &L : s->iter_sets)
             if (voce_launch_iteration(&L, s->stream, nullptr)) return VOCE_ERR_DEVICE;
     for (LaunchSet &L : s->iter_sets)   // the chunk's last Newton step
         if (voce_launch_finalize(&L, s->stream)) return VOCE_ERR_DEVICE;
-    s->iters += s->o.chunk;
+    for (Inflight &f : s->inflight) f.launched += chunk;
+    s->iters += chunk;
     // 4. poll and close out the finished rows
     for (size_t i = 0; i < rows.size(); ++i) s->h_idx[i] = rows[i];
     SCK(hipMemcpyAsync(s->d_idx, s->h_idx, rows.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
@@ -745,6 +759,7 @@ static int server_step(voce_server *s)
             if ((rc = upload_row(s, f.row))) return rc;
             s->busy[f.row] = 0;
             c->n_steps_done = M.istep;
+            s->pred[f.req.id] = M.iter > 0 ? M.iter : 1;
             voce_result res;
             memset(&res, 0, sizeof(res));
             res.tag = f.req.tag;
