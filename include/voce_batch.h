@@ -85,7 +85,7 @@ typedef struct voce_server_opts {
     int precision;            /* VOCE_MIXED or VOCE_FP64 for served increments */
     int max_rves;             /* ids that can be registered (table capacity) */
     int max_inflight;         /* increments iterating at once (0: memory-bound only) */
-    size_t workspace_bytes;   /* workspace pool (0: a quarter of the free device memory) */
+    size_t workspace_bytes;   /* workspace pool (0: an eighth of the free device memory, at most 16 GiB) */
     int tune;                 /* autotune each grid size when first registered */
     int chunk;                /* most FFT iterations launched between polls (0: 16); a round
                                  launches up to the nearest predicted end of an increment in

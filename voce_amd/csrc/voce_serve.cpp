@@ -245,9 +245,12 @@ extern "C" int voce_server_create(const voce_server_opts *opts, voce_server **ou
     for (int i = 0; i < s->max_shadow; ++i) s->shadows[i].row = s->o.max_rves + i;
     size_t wsb = s->o.workspace_bytes;
     if (wsb == 0) {
+        // default pool: an eighth of the free memory, at most 16 GiB (several
+        // hundred 64^3 increments in flight), leaving the rest for resident ids
         size_t fr = 0, tot = 0;
         (void)hipMemGetInfo(&fr, &tot);
-        wsb = fr / 4;
+        wsb = fr / 8;
+        if (wsb > ((size_t)16 << 30)) wsb = (size_t)16 << 30;
     }
     wsb = al256(wsb);
     if (hipMalloc((void **)&s->pool_mem, wsb) != hipSuccess) FAIL(VOCE_ERR_NOMEM);
