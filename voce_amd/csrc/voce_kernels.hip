@@ -11,13 +11,15 @@
 //                real z FFT, and the strain update
 //                E += dDisGradAvg + IFFT/N (evolution.c:383-394).
 //   k_nr         one voxel per lane: the 6x6 Newton-Raphson of UpdateStress
-//                (evolution.c:3-168) with StrainRate_eval (constitutive.c:4-72)
-//                evaluated in the crystal frame and the LU of LU_inv_66
-//                (kinematics.c:282-424) in registers, wavefront-shuffle
-//                partial sums of the error norms and stress averages; the
-//                last block to finish sums them in fixed order and runs
-//                get_smacro (evolution.c:170-241), the error normalisation
-//                and the convergence test (evolution.c:342, 405-428).
+//                (evolution.c:3-168) with StrainRate_eval (constitutive.c:4-72),
+//                solved in compliance form by an LDL^T factorisation in
+//                registers (in place of LU_inv_66, kinematics.c:282-424);
+//                wavefront-shuffle partial sums of the error norms and
+//                stress averages per block.  One extra block of the next
+//                forward-plane launch (or k_fin_iter after the last launched
+//                iteration) sums them in fixed order and runs get_smacro
+//                (evolution.c:170-241), the error normalisation and the
+//                convergence test (evolution.c:342, 405-428).
 // Every launch reads MacroState::active first, so iterations enqueued past
 // convergence return at once and the host never blocks inside a loop.
 //
@@ -44,6 +46,11 @@ constexpr double TWO_PI_REF = 6.28318530718;    // evp.h:58
 // writes while the view is in use.
 template <typename T> using cptr = const __attribute__((address_space(4))) T *;
 template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+// Global-address-space view of a device pointer: vector loads through it are
+// global (not flat) loads, which count against the vector-memory counter
+// only, so waiting on them does not also wait on LDS and scalar loads.
+template <typename T> using gptr = const __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)p; }
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<double> { using t = double2; };
@@ -1039,8 +1046,8 @@ __device__ __forceinline__ double rcp_refined(double d)
 // major) plus dt D (D packed upper 5x5, zero 6th row/column) by LDL^T.
 // Returns 35 (the reference's singular-matrix code) if a pivot is not
 // positive, which a positive definite matrix never gives.
-__device__ __forceinline__ int ldlt_solve6(const double *Kc, const double D[15], double dt, const double g[6],
-                                           double x[6])
+template <typename P>
+__device__ __forceinline__ int ldlt_solve6(P Kc, const double D[15], double dt, const double g[6], double x[6])
 {
     double L[6][6], Dd[6], iD[6];
     int bad = 0;
@@ -1138,7 +1145,7 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
             const cptr<MacroState> msc = as_const(ms);
             const double dt = msc->dt;
             const cptr<double> C0 = msc->C066;
-            const double *tab = r.SLOC + (long)ori * VOCE_OTAB;   // S_local, A^-1, A^-1 C0
+            const gptr<double> tab = as_global(r.SLOC + (long)ori * VOCE_OTAB);   // S_local, K_c, initial TranMat
             // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
             // it would form per evaluation)
 #pragma unroll
@@ -1235,7 +1242,13 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                     double pn1, pn;
                     if (intn) {
                         const double ar = fabs(rr);
-                        pn1 = ipow(ar, ni - 1);
+                        if (ni == 20) {
+                            // x^19 with ipow's products in ipow's order
+                            const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+                            pn1 = (ar * x2) * x16;
+                        } else {
+                            pn1 = ipow(ar, ni - 1);
+                        }
                         pn = pn1 * ar;
                     } else {
                         pn1 = fabs(pow(rr, nsr - 1.));
@@ -1260,8 +1273,11 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 // factor C0 = C066: J = C0 K with K = K_c + dt D, K_c = S0 + S_local,
                 // and C0^-1 R = g = K_c sigma + w + dt edot.  K is symmetric positive
                 // definite (two compliances plus the PSD flow-rule Jacobian), so
-                // x = K^-1 g is an LDL^T solve without pivoting.
-                const double *Kc = tab + 36;
+                // x = K^-1 g is an LDL^T solve without pivoting.  g is formed from
+                // sigma afresh every step: a running update of K_c sigma would
+                // carry the rounding of the large early steps (where dt edot and
+                // dt D x nearly cancel) into the converged stress.
+                const gptr<double> Kc = tab + 36;
                 double g[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
