@@ -38,6 +38,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 
 
+PROF_EVERY = 10
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -81,7 +84,9 @@ def run_gpu(path, base, precision, steps, warmup, prof=True, tune=True):
     if tuned:
         log(precision, "tuned FFT launch choice", tuned)
     if prof:
-        s.prof_enable(True)
+        # events on every PROF_EVERY-th FFT iteration: each recorded event
+        # costs queue time, a sparse sample keeps the timed run honest
+        s.prof_enable(True, every=PROF_EVERY)
     reps = []
     t0 = time.perf_counter()
     for k in range(steps):

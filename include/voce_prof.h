@@ -4,8 +4,12 @@
  * No counterpart in the reference (it has no timers, see evolution.c:344-348
  * for its only progress output).  For benchmarks that must report a
  * kernel's own average duration: HIP events are recorded on the context's
- * stream between the five launches of every FFT iteration, and only
- * iterations that did work (before convergence) are accumulated.
+ * stream between the launches of every k-th FFT iteration (k chosen at
+ * enable; each recorded event costs a few microseconds of queue time, so a
+ * sparse sample keeps the timed run close to an untimed one).  The sampled
+ * residue shifts by one per increment, so k consecutive increments sample
+ * every iteration index once.  Only iterations that did work (before
+ * convergence) are accumulated.
  */
 #ifndef VOCE_PROF_H
 #define VOCE_PROF_H
@@ -26,7 +30,8 @@ extern "C" {
                                    forward-plane launch) */
 #define VOCE_PROF_NK 5
 
-/* Turn timing on (1) or off (0); either way the totals are reset. */
+/* Timing off (0), or on for every on-th FFT iteration of each increment
+ * (1: every iteration); either way the totals are reset. */
 int voce_prof_enable(voce_ctx *ctx, int on);
 /* Totals since the last enable: ms_total[k] milliseconds over count[k] launches. */
 int voce_prof_read(voce_ctx *ctx, double ms_total[VOCE_PROF_NK], long count[VOCE_PROF_NK]);

@@ -318,8 +318,10 @@ class Solver:
         if rc:
             raise VoceError(rc, "voce_set_velgrad")
 
-    def prof_enable(self, on=True):
-        load_library().voce_prof_enable(self._h, int(bool(on)))
+    def prof_enable(self, on=True, every=1):
+        """Kernel timing with HIP events on every `every`-th FFT iteration
+        of each increment (off when `on` is false)."""
+        load_library().voce_prof_enable(self._h, max(1, int(every)) if on else 0)
 
     def prof_read(self) -> dict:
         ms = (C.c_double * 5)()

@@ -792,8 +792,12 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             c->ev.resize(6 * n);
             for (size_t i = old; i < c->ev.size(); ++i) CK(hipEventCreate(&c->ev[i]));
         }
+        // timed iterations: every prof-th FFT iteration, the residue moving
+        // by one per increment (early iterations of an increment run more
+        // Newton steps; over prof increments every iteration index is sampled)
+        auto timed = [&](int it) { return c->prof > 0 && (it + c->n_steps_done) % c->prof == 0; };
         for (int i = 0; i < n; ++i) {
-            e = voce_launch_iteration(&c->L, c->stream, c->prof ? (void *)&c->ev[6 * i] : nullptr);
+            e = voce_launch_iteration(&c->L, c->stream, timed(launched + i) ? (void *)&c->ev[6 * i] : nullptr);
             if (e) return VOCE_ERR_DEVICE;
         }
         launched += n;
@@ -805,7 +809,8 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
         if (c->prof) {
             // only the iterations that ran (the rest returned at the flag check)
             const int ran = c->h_ctl[1] - done_before;
-            for (int i = 0; i < ran && i < n; ++i)
+            for (int i = 0; i < ran && i < n; ++i) {
+                if (!timed(launched - n + i)) continue;
                 for (int k = 0; k < VOCE_PROF_FIN; ++k) {   // events ev[0..4] bracket the 4 kernels
                     float ms = 0.f;
                     if (hipEventElapsedTime(&ms, c->ev[6 * i + k], c->ev[6 * i + k + 1]) == hipSuccess) {
@@ -813,6 +818,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
                         c->prof_n[k] += 1;
                     }
                 }
+            }
             done_before = c->h_ctl[1];
         }
         if (!c->h_ctl[3]) break;
@@ -1050,7 +1056,7 @@ extern "C" int voce_write_fields(voce_ctx *c, const char *dir, int step, int tex
 extern "C" int voce_prof_enable(voce_ctx *c, int on)
 {
     if (!c) return VOCE_ERR_ARG;
-    c->prof = on ? 1 : 0;
+    c->prof = on > 0 ? on : 0;
     for (int k = 0; k < VOCE_PROF_NK; ++k) {
         c->prof_ms[k] = 0.0;
         c->prof_n[k] = 0;

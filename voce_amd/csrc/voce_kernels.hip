@@ -524,7 +524,21 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     const int tid = threadIdx.x;
     const C *gtx = (const C *)r.tw_x;
     for (int i = tid; i < nx; i += NT) twx[i] = gtx[i];
-    const int col0 = lb * tk;
+    // Tiles that share 128-byte lines run on one XCD, whose L2 then fetches
+    // each line once: workgroups go to the 8 XCDs round robin, so local
+    // blocks lb, lb + 8, ..., lb + 8 (G - 1) (one XCD) take the G = TB / tk
+    // tiles of one line group.  Blocks past the last whole span of 8 G keep
+    // tile = lb.
+    int tile = lb;
+    {
+        const int G = TB / tk, span = 8 * G;
+        const int nt = (ncolt + tk - 1) / tk;
+        if (G > 1 && lb < (nt / span) * span) {
+            const int xcd = lb & 7, j = lb >> 3;
+            tile = ((j / G) * 8 + xcd) * G + (j % G);
+        }
+    }
+    const int col0 = tile * tk;
     const int ncol = min(tk, ncolt - col0);
     const bool first = (ms->iter == 0);
     auto gofs = [&](int cc, int x, int col) -> long {
