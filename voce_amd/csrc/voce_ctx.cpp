@@ -474,12 +474,16 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     c66_to_c4(S066, S0);
     // per-orientation table (stride VOCE_OTAB): S_local = C_gr^-1, then
     // K_c = C066^-1 + S_local, the constant part of the compliance-form
-    // Newton matrix of UpdateStress (evolution.c:85-93 with C066 factored out)
+    // Newton matrix of UpdateStress (evolution.c:85-93 with C066 factored out).
+    // K_c is symmetric up to the rounding of the two inverses; it is stored
+    // as the mean of itself and its transpose, so the Newton kernel reads one
+    // triangle for both the residual and the LDL^T factorisation.
     std::vector<double> otab(VOCE_OTAB * (norient > 0 ? norient : 1), 0.0);
     for (long o = 0; o < norient; ++o) {
         for (int i = 0; i < 36; ++i) {
+            const int a = i / 6, b = i % 6;
             otab[VOCE_OTAB * o + i] = Sloc[36 * o + i];
-            otab[VOCE_OTAB * o + 36 + i] = S066[i / 6][i % 6] + Sloc[36 * o + i];
+            otab[VOCE_OTAB * o + 36 + i] = 0.5 * ((S066[a][b] + Sloc[36 * o + i]) + (S066[b][a] + Sloc[36 * o + b * 6 + a]));
         }
         for (int i = 0; i < 9; ++i) otab[VOCE_OTAB * o + 72 + i] = Rtab[o][i];   // initial TranMat
     }

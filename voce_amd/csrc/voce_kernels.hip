@@ -51,6 +51,30 @@ template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { 
 // only, so waiting on them does not also wait on LDS and scalar loads.
 template <typename T> using gptr = const __attribute__((address_space(1))) T *;
 template <typename T> __device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)p; }
+template <typename T> using gmut = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gmut<T> as_global_mut(T *p) { return (gmut<T>)p; }
+
+// Strided element loop e = tid, tid + NT, ... < n whose global loads go out
+// U at a time: v = ld(e) for a group of U elements, then st(e, v) for each.
+// (In a plain loop every load is waited for before the next is issued, so
+// a block spends one memory latency per element it stages.)
+template <int NT, int U, typename V, typename LD, typename ST>
+__device__ __forceinline__ void grouped(int n, LD ld, ST st)
+{
+    for (int e0 = (int)threadIdx.x; e0 < n; e0 += U * NT) {
+        V v[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            const int e = e0 + i * NT;
+            if (e < n) v[i] = ld(e);
+        }
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            const int e = e0 + i * NT;
+            if (e < n) st(e, v[i]);
+        }
+    }
+}
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<double> { using t = double2; };
@@ -276,12 +300,12 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
     const C *gtz = (const C *)r.tw_z;
     for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
     for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
-    const double *src = r.S6 + (long)c * r.V + (long)x * ny * nz;
+    const gptr<double> src = as_global(r.S6 + (long)c * r.V + (long)x * ny * nz);
     T *sr = (T *)s;
-    for (int e = tid; e < ny * nz; e += NT) {
+    grouped<NT, 16, double>(ny * nz, [&](int e) { return src[e]; }, [&](int e, double v) {
         const int y = e / nz, z = e - y * nz;
-        sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)] = (T)src[e];
-    }
+        sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)] = (T)v;
+    });
     __syncthreads();
     fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, 2 * nzh, 1, twz, r.pl_r8);
     // split packed rows a + i b into the half spectra of a and b
@@ -343,7 +367,8 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
     constexpr int TB = 128 / (int)sizeof(C);
     const C *srcp = (const C *)r.spec + (long)c * r.nspec;
-    for (int e = tid; e < ny * nzh; e += NT) s[e] = srcp[((long)(e / TB) * nx + x) * TB + (e % TB)];
+    grouped<NT, 8, C>(ny * nzh, [&](int e) { return srcp[((long)(e / TB) * nx + x) * TB + (e % TB)]; },
+                      [&](int e, C v) { s[e] = v; });
     __syncthreads();
     fft_lines<C, NT, MAXEL, true, true>(s, nzh, ny, 1, nzh, twy, r.pl_r8);
     // pack the half spectra of rows 2p (a) and 2p+1 (b) into a full complex
@@ -387,16 +412,15 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     const long off = (long)c * r.V + (long)x * ny * nz;
     if (mode == 0) {
         const MacroState *ms = r.ms;
-        double *E = r.E6 + off;
+        const gmut<double> E = as_global_mut(r.E6 + off);
         const bool creep_set = (ms->creep != 0) && (ms->iter == 0);
         const double sh = creep_set ? 0.0 : ms->shift6[c];
         const double cr = ms->creep_sym6[c];
-        for (int e = tid; e < ny * nz; e += NT) {
+        grouped<NT, 16, double>(ny * nz, [&](int e) { return creep_set ? 0.0 : E[e]; }, [&](int e, double ev) {
             const int y = e / nz, z = e - y * nz;
             const double val = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
-            if (creep_set) E[e] = cr + (sh + val);
-            else E[e] = E[e] + (sh + val);
-        }
+            E[e] = creep_set ? cr + (sh + val) : ev + (sh + val);
+        });
     } else {
         double *W = r.WF + off;
         for (int e = tid; e < ny * nz; e += NT) {
@@ -493,6 +517,48 @@ __device__ __forceinline__ void green_apply(const cptr<T> G, const T n[3], const
     w[2] = C{(T)0.5 * (u10.x - u01.x), (T)0.5 * (u10.y - u01.y)};
 }
 
+// x-pass tile staging.  Element e = (x, t) of a tile of tk (a power of two,
+// 2^lgt) columns starting at col0, ncol of them valid; component cc of the
+// column-blocked spectrum at cc * cstr + ((col / TB) * nx + x) * TB + col % TB,
+// in LDS at line (cc * tk + t), padded to nx + 1.  All NC components of an
+// element are loaded before any is stored.
+template <int NC, int NT, typename C>
+__device__ __forceinline__ void xtile_in(const C *__restrict__ g, C *__restrict__ l, long cstr, int nx, int tk,
+                                         int lgt, int ncol, int col0)
+{
+    constexpr int TB = 128 / (int)sizeof(C);
+    const int per = nx << lgt, NXP = nx + 1;
+    for (int e = (int)threadIdx.x; e < per; e += NT) {
+        const int x = e >> lgt, t = e & (tk - 1);
+        if (t < ncol) {
+            const int col = col0 + t;
+            const long o = ((long)(col / TB) * nx + x) * TB + (col % TB);
+            C v[NC];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) v[cc] = g[o + cc * cstr];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) l[(cc * tk + t) * NXP + x] = v[cc];
+        }
+    }
+}
+
+template <int NC, int NT, typename C>
+__device__ __forceinline__ void xtile_out(const C *__restrict__ l, C *__restrict__ g, long cstr, int nx, int tk,
+                                          int lgt, int ncol, int col0)
+{
+    constexpr int TB = 128 / (int)sizeof(C);
+    const int per = nx << lgt, NXP = nx + 1;
+    for (int e = (int)threadIdx.x; e < per; e += NT) {
+        const int x = e >> lgt, t = e & (tk - 1);
+        if (t < ncol) {
+            const int col = col0 + t;
+            const long o = ((long)(col / TB) * nx + x) * TB + (col % TB);
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) g[o + cc * cstr] = l[(cc * tk + t) * NXP + x];
+        }
+    }
+}
+
 // ------------------------------------------------------------------
 // x-direction FFT + Green operator.  grid ceil(ny*nzh / tk)
 // mode 0 (iteration): SPEC (6 comps) -> fwd x FFT -> Delta = B(sym(u (x) n))
@@ -541,24 +607,15 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     const int col0 = tile * tk;
     const int ncol = min(tk, ncolt - col0);
     const bool first = (ms->iter == 0);
-    auto gofs = [&](int cc, int x, int col) -> long {
-        return ((long)(cc * ncb + col / TB) * nx + x) * TB + (col % TB);
-    };
     const C *src = (const C *)((mode == 0) ? r.spec : r.usum);
     C *usum = (C *)r.usum;
-    for (int e = tid; e < ncomp * nx * ncol; e += NT) {
-        const int cc = e / (nx * ncol);
-        const int rem = e - cc * nx * ncol;
-        const int x = rem / ncol, t = rem - x * ncol;
-        s[(cc * tk + t) * NXP + x] = src[gofs(cc, x, col0 + t)];
-    }
-    if (mode == 0 && !first) {
-        for (int e = tid; e < 3 * nx * ncol; e += NT) {
-            const int cc = e / (nx * ncol);
-            const int rem = e - cc * nx * ncol;
-            const int x = rem / ncol, t = rem - x * ncol;
-            u[(cc * tk + t) * NXP + x] = usum[gofs(cc, x, col0 + t)];
-        }
+    const int lgt = 31 - __builtin_clz(tk);   // tk is a power of two
+    const long cstr = (long)ncb * nx * TB;     // one component (= nspec)
+    if (mode == 0) {
+        xtile_in<6, NT>(src, s, cstr, nx, tk, lgt, ncol, col0);
+        if (!first) xtile_in<3, NT>(usum, u, cstr, nx, tk, lgt, ncol, col0);
+    } else {
+        xtile_in<3, NT>(src, s, cstr, nx, tk, lgt, ncol, col0);
     }
     __syncthreads();
     if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, NXP, 1, twx, r.x_r8);
@@ -614,19 +671,11 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     __syncthreads();
     fft_lines<C, NT, MAXEL, true, false>(s, ncomp * tk, nx, NXP, 1, twx, r.x_r8);
     C *dst = (C *)r.spec;
-    for (int e = tid; e < ncomp * nx * ncol; e += NT) {
-        const int cc = e / (nx * ncol);
-        const int rem = e - cc * nx * ncol;
-        const int x = rem / ncol, t = rem - x * ncol;
-        dst[gofs(cc, x, col0 + t)] = s[(cc * tk + t) * NXP + x];
-    }
     if (mode == 0) {
-        for (int e = tid; e < 3 * nx * ncol; e += NT) {
-            const int cc = e / (nx * ncol);
-            const int rem = e - cc * nx * ncol;
-            const int x = rem / ncol, t = rem - x * ncol;
-            usum[gofs(cc, x, col0 + t)] = u[(cc * tk + t) * NXP + x];
-        }
+        xtile_out<6, NT>(s, dst, cstr, nx, tk, lgt, ncol, col0);
+        xtile_out<3, NT>(u, usum, cstr, nx, tk, lgt, ncol, col0);
+    } else {
+        xtile_out<3, NT>(s, dst, cstr, nx, tk, lgt, ncol, col0);
     }
 }
 
@@ -1056,32 +1105,34 @@ __device__ __forceinline__ double rcp_refined(double d)
     return fma(r, e, r);
 }
 
-// Solve (K_c + dt D) x = g for symmetric positive definite K_c (6x6, row
-// major) plus dt D (D packed upper 5x5, zero 6th row/column) by LDL^T.
+// Packed lower triangle of a symmetric 6x6: (i, j), j <= i, at i (i + 1) / 2 + j.
+__device__ __forceinline__ constexpr int lt6(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Solve K x = g for symmetric positive definite K (packed lower triangle,
+// factorised in place: L below the diagonal, the pivots on it) by LDL^T.
 // Returns 35 (the reference's singular-matrix code) if a pivot is not
 // positive, which a positive definite matrix never gives.
-template <typename P>
-__device__ __forceinline__ int ldlt_solve6(P Kc, const double D[15], double dt, const double g[6], double x[6])
+__device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6], double x[6])
 {
-    double L[6][6], Dd[6], iD[6];
+    double iD[6];
     int bad = 0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         double v[6];
 #pragma unroll
-        for (int k = 0; k < j; ++k) v[k] = L[j][k] * Dd[k];
-        double d = Kc[j * 6 + j] + (j < 5 ? dt * Dget(D, j, j) : 0.0);
+        for (int k = 0; k < j; ++k) v[k] = K[lt6(j, k)] * K[lt6(k, k)];
+        double d = K[lt6(j, j)];
 #pragma unroll
-        for (int k = 0; k < j; ++k) d -= L[j][k] * v[k];
+        for (int k = 0; k < j; ++k) d -= K[lt6(j, k)] * v[k];
         bad |= !(d > 0.0);
-        Dd[j] = d;
+        K[lt6(j, j)] = d;
         iD[j] = rcp_refined(d);
 #pragma unroll
         for (int i = j + 1; i < 6; ++i) {
-            double t = Kc[i * 6 + j] + (i < 5 && j < 5 ? dt * Dget(D, i, j) : 0.0);
+            double t = K[lt6(i, j)];
 #pragma unroll
-            for (int k = 0; k < j; ++k) t -= L[i][k] * v[k];
-            L[i][j] = t * iD[j];
+            for (int k = 0; k < j; ++k) t -= K[lt6(i, k)] * v[k];
+            K[lt6(i, j)] = t * iD[j];
         }
     }
     double y[6];
@@ -1089,14 +1140,14 @@ __device__ __forceinline__ int ldlt_solve6(P Kc, const double D[15], double dt, 
     for (int i = 0; i < 6; ++i) {
         double t = g[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+        for (int k = 0; k < i; ++k) t -= K[lt6(i, k)] * y[k];
         y[i] = t;
     }
 #pragma unroll
     for (int i = 5; i >= 0; --i) {
         double t = y[i] * iD[i];
 #pragma unroll
-        for (int k = i + 1; k < 6; ++k) t -= L[k][i] * x[k];
+        for (int k = i + 1; k < 6; ++k) t -= K[lt6(k, i)] * x[k];
         x[i] = t;
     }
     return bad ? 35 : 0;
@@ -1291,17 +1342,28 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 // sigma afresh every step: a running update of K_c sigma would
                 // carry the rounding of the large early steps (where dt edot and
                 // dt D x nearly cancel) into the converged stress.
+                // K_c is stored symmetric: one triangle (21 loads, issued
+                // together) serves the residual and the factorisation
                 const gptr<double> Kc = tab + 36;
+                double K[21];
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) K[lt6(i, j)] = Kc[i * 6 + j];
                 double g[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                     double t = lds_w[i][threadIdx.x] + (i < 5 ? dt * ed[i] : 0.0);
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) t += Kc[i * 6 + j] * sig[j];
+                    for (int j = 0; j < 6; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
                     g[i] = t;
                 }
+#pragma unroll
+                for (int i = 0; i < 5; ++i)
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) K[lt6(i, j)] += dt * Dget(D, i, j);
                 double x[6];
-                if (ldlt_solve6(Kc, D, dt, g, x)) {
+                if (ldlt_solve6_packed(K, g, x)) {
                     ms->error = 35;
                     break;
                 }
@@ -1919,9 +1981,11 @@ static const size_t LDS_MAX = 160 * 1024;
 // Largest x-pass tile (columns per block, <= 16) for variant xv.
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
-    int tk = X_MAXEL[xv] / (6 * nx);
-    if (tk > 16) tk = 16;
-    return tk;   // 0: the variant cannot hold one column
+    const int fit = X_MAXEL[xv] / (6 * nx);
+    if (fit < 1) return 0;   // the variant cannot hold one column
+    int tk = 1;
+    while (2 * tk <= fit && 2 * tk <= 16) tk *= 2;
+    return tk;               // the largest power of two <= 16 that fits
 }
 
 // Dynamic LDS of the plane and x-pass kernels of one RVE under its tile
@@ -1940,7 +2004,7 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 
 extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
 {
-    if (xv < 0 || xv > 2 || tk < 1 || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
+    if (xv < 0 || xv > 2 || tk < 1 || (tk & (tk - 1)) || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
     // 6 spectra + 3 skew sums, lines padded to nx + 1, then the twiddles
     const size_t b = (size_t)(9 * tk * (r->nx + 1) + r->nx) * cb;
