@@ -36,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+FP64_PEAK_TFS = 78.6    # MI355X FP64 vector
+PATH_BYTES_PER_VOXEL_ITER = 768   # FP64: FFT 96 + Green/inverse 96 + Newton reads 432 + writes 144
 
 
 PROF_EVERY = 10
@@ -279,12 +281,13 @@ def main():
     achieved = bpv * V / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     # measured HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE/WRITE_SIZE
     # (scaled by a copy-kernel calibration) committed with the profile summary
-    traffic, tsrc = None, None
+    traffic, tsrc, flop = None, None, None
     tf = os.path.join(ROOT, "profiles", "iteration_kernels_64cube_fp64.traffic.json")
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
             traffic = tj["bytes_per_launch"].get(dom)
+            flop = tj.get("fp64_flop_per_launch", {}).get(dom)
             tsrc = "profiles/iteration_kernels_64cube_fp64.md"
         except Exception:
             traffic = None
@@ -292,6 +295,21 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": (round(traffic) if traffic else None),
             "traffic_source": tsrc, "algorithmic_bytes": round(bpv * V),
             "bytes_per_voxel": round(bpv, 2), "avg_launch_ms": round(avg_ms, 5)}
+    # The Newton kernel is FP64-VALU bound rather than HBM bound: its FP64
+    # work per launch (rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, 64
+    # lanes per wave instruction, FMA = 2) over the live average duration,
+    # against the MI355X FP64 vector peak
+    roof_fp64 = None
+    if flop and avg_ms > 0:
+        tf_s = flop / (avg_ms * 1e-3) / 1e12
+        roof_fp64 = {"bound": "fp64-valu", "kernel": dom, "achieved": round(tf_s, 2), "peak": FP64_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": round(tf_s / FP64_PEAK_TFS, 4), "flop_per_launch": round(flop),
+                     "flop_source": tsrc}
+    # whole FFT iteration at the per-voxel-iteration byte count of the
+    # reference path's single-pass design (768 B FP64, every operand once)
+    path_gbs = vox_it * PATH_BYTES_PER_VOXEL_ITER / 1e9
+    roof_path = {"bytes_per_voxel_iteration": PATH_BYTES_PER_VOXEL_ITER, "achieved": round(path_gbs, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)}
     per_kernel = {k: {"avg_ms": round(v[0], 5), "launches": v[1], "share": round(v[2] / max(1e-12, sum(x[2] for x in kern.values())), 4)}
                   for k, v in kern.items()}
 
@@ -315,6 +333,8 @@ def main():
         "fft_iterations": iters,
         "increment_ms_p50": None,
         "roofline": roof,
+        "roofline_fp64": roof_fp64,
+        "roofline_path": roof_path,
         "kernels": per_kernel,
         "fft_launch_choice": run_gpu.tuned,
     }

@@ -1426,9 +1426,10 @@ __device__ __forceinline__ void finalize_block(const RveDev &r)
     __shared__ MacroState sms;
     static_assert(sizeof(MacroState) % 8 == 0, "MacroState is copied as doubles");
     {
-        const double *src = (const double *)ms;
+        const gptr<double> src = as_global((const double *)ms);
         double *dst = (double *)&sms;
-        for (int i = threadIdx.x; i < (int)(sizeof(MacroState) / 8); i += NT) dst[i] = src[i];
+        grouped<NT, 4, double>((int)(sizeof(MacroState) / 8), [&](int i) { return src[i]; },
+                               [&](int i, double v) { dst[i] = v; });
     }
     __shared__ double tot[16];
     sum_partials<NT, 14>(r.part, r.nblk_vox, tot);   // ends with a barrier
