@@ -993,65 +993,64 @@ __device__ double svm_of(const double S[9])
 }
 
 // Finaliser of one iteration (evolution.c:405-428 and get_smacro,
-// evolution.c:170-241), run by thread 0 on the summed partials.
-__device__ void finalize_iteration(MacroState *ms, const MacroState *in, const double *tot)
+// evolution.c:170-241), run by thread 0 on the summed partials.  `in` is
+// the LDS copy of the macro state: its entries are read where they are
+// used, so the finaliser holds few registers (it shares its kernel's
+// register allocation with the forward plane blocks, whose occupancy it
+// would otherwise set).
+__device__ void finalize_iteration(MacroState *ms, const MacroState *in, const double *tot, double *lds_dD)
 {
-    // all reads first, then the arithmetic, then all writes: loads and
-    // stores share one wait counter, so interleaving them would serialise
-    // every access
-    const double wgt = in->wgt, wgt1 = in->wgt_over_ph1, e_vm = in->e_vm, err_tol = in->err_tol;
-    const int it = in->iter, itot = in->iter_total, itmax = in->iter_max;
-    int vfl[6], sfl[6];
-    double Sc[9], acum[9], applied[9], S0s[6][6];
-    const int IJV0[6] = {0, 1, 2, 1, 0, 0}, IJV1[6] = {0, 1, 2, 2, 2, 1};
-    for (int i = 0; i < 6; ++i) {
-        vfl[i] = in->velgrad_flag[i];
-        sfl[i] = in->stress_flag[i];
-        for (int k = 0; k < 6; ++k)
-            S0s[i][k] = in->S0[((IJV0[i] * 3 + IJV1[i]) * 3 + IJV0[k]) * 3 + IJV1[k]];
+    const double wgt = in->wgt, wgt1 = in->wgt_over_ph1;
+    // IJV (evolution.c:179) as nibbles: Voigt i -> (IJV0, IJV1), so a rolled
+    // loop indexes no private array
+    constexpr unsigned IJV0 = 0x001210u, IJV1 = 0x122210u;
+    auto ij0 = [](int i) { return (int)((IJV0 >> (4 * i)) & 15u); };
+    auto ij1 = [](int i) { return (int)((IJV1 >> (4 * i)) & 15u); };
+    double SA[9], SA1[9];
+    {
+        double s6[6], s16[6];
+        for (int i = 0; i < 6; ++i) { s6[i] = tot[2 + i]; s16[i] = tot[8 + i]; }
+        b6_to_t(s6, SA);
+        b6_to_t(s16, SA1);
     }
-    for (int i = 0; i < 9; ++i) {
-        Sc[i] = in->Scauchy[i];
-        acum[i] = in->acum[i];
-        applied[i] = in->applied[i];
-    }
-    double s6[6], s16[6], SA[9], SA1[9];
-    for (int i = 0; i < 6; ++i) { s6[i] = tot[2 + i]; s16[i] = tot[8 + i]; }
-    b6_to_t(s6, SA);
-    b6_to_t(s16, SA1);
     for (int i = 0; i < 9; ++i) {
         SA[i] *= wgt;
         SA1[i] *= wgt1;
     }
-    // mixed-BC correction, upper triangle only (evolution.c:199-214)
-    double dD[9];
-    for (int i = 0; i < 9; ++i) dD[i] = 0.0;
+    // mixed-BC correction, upper triangle only (evolution.c:199-214); the
+    // rolled row loop collects dD in LDS
+    double rhs[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int kk = ij0(k), ll = ij1(k);
+        rhs[k] = in->Scauchy[kk * 3 + ll] - SA[kk * 3 + ll];
+    }
+    for (int i = 0; i < 9; ++i) lds_dD[i] = 0.0;
+#pragma unroll 1
     for (int i = 0; i < 6; ++i) {
         double acc = 0.0;
-        if (vfl[i] == 0) {
-            for (int k = 0; k < 6; ++k) {
-                const int kk = IJV0[k], ll = IJV1[k];
-                acc += S0s[i][k] * sfl[k] * (Sc[kk * 3 + ll] - SA[kk * 3 + ll]);
-            }
+        if (in->velgrad_flag[i] == 0) {
+            const int row = (ij0(i) * 3 + ij1(i)) * 9;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                acc += in->S0[row + ij0(k) * 3 + ij1(k)] * in->stress_flag[k] * rhs[k];
         }
-        dD[IJV0[i] * 3 + IJV1[i]] = acc;
+        lds_dD[ij0(i) * 3 + ij1(i)] = acc;
     }
-    for (int i = 0; i < 9; ++i) acum[i] += dD[i];
+    double dD[9];
+    for (int i = 0; i < 9; ++i) dD[i] = lds_dD[i];
     const double s_vm = svm_of(SA), s_vm1 = svm_of(SA1);
+    const double e_vm = in->e_vm;
     double Err_e = tot[1] * wgt, Err_s = tot[0] * wgt;
     if (fabs(e_vm) > 1E-5) Err_e /= e_vm;
     Err_s /= s_vm;
-    const int more = (it + 1 < itmax) && (fmax(fabs(Err_s), fabs(Err_e)) > fabs(err_tol));
-    double sh[6];
-    if (more) {
-        for (int i = 0; i < 9; ++i) applied[i] += dD[i];
-        sym_to_b6(dD, sh);
-    }
+    const int it = in->iter;
+    const int more = (it + 1 < in->iter_max) && (fmax(fabs(Err_s), fabs(Err_e)) > fabs(in->err_tol));
     for (int i = 0; i < 9; ++i) {
         ms->SigAvg[i] = SA[i];
         ms->SigAvg1[i] = SA1[i];
         ms->dD[i] = dD[i];
-        ms->acum[i] = acum[i];
+        ms->acum[i] = in->acum[i] + dD[i];
     }
     ms->s_vm = s_vm;
     ms->s_vm1 = s_vm1;
@@ -1065,9 +1064,11 @@ __device__ void finalize_iteration(MacroState *ms, const MacroState *in, const d
         lg[2 * cap + it] = s_vm;
     }
     ms->iter = it + 1;
-    ms->iter_total = itot + 1;
+    ms->iter_total = in->iter_total + 1;
     if (more) {
-        for (int i = 0; i < 9; ++i) ms->applied[i] = applied[i];
+        for (int i = 0; i < 9; ++i) ms->applied[i] = in->applied[i] + dD[i];
+        double sh[6];
+        sym_to_b6(dD, sh);
         for (int i = 0; i < 6; ++i) ms->shift6[i] = sh[i];
     }
     ms->active = more;   // read by the next kernels (the launch boundary orders it)
@@ -1432,9 +1433,10 @@ __device__ __forceinline__ void finalize_block(const RveDev &r)
                                [&](int i, double v) { dst[i] = v; });
     }
     __shared__ double tot[16];
+    __shared__ double dDs[9];
     sum_partials<NT, 14>(r.part, r.nblk_vox, tot);   // ends with a barrier
     if (threadIdx.x == 0) {
-        finalize_iteration(ms, &sms, tot);
+        finalize_iteration(ms, &sms, tot, dDs);
         ms->fin_pending = 0;
     }
 }
