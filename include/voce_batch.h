@@ -21,7 +21,8 @@ extern "C" {
 
 /* Launch choices of the three FFT kernels for one grid size and precision:
  * plane-kernel variant (0: 64 threads / planes <= 32x32, 1: 256 / <= 64x64,
- * 2: 512 / <= 128x128), x-pass variant (0: 256 threads, 1: 128, 2: 64),
+ * 2: 512 / <= 128x128, 3: 320 / <= 64x64), x-pass variant (0: 256 threads,
+ * 1: 128, 2: 64),
  * x-pass tile (columns per block), and whether radix-8 stages are used in
  * the plane and x passes.  us_* are the measured launch times (microseconds)
  * of the chosen plane pair (forward + inverse) and x pass. */
@@ -38,14 +39,18 @@ typedef struct voce_tune_choice {
 int voce_autotune(voce_ctx *ctx, voce_tune_choice *out);
 /* Apply a known choice (e.g. from a file) without timing. */
 int voce_apply_tuning(voce_ctx *ctx, const voce_tune_choice *choice);
+/* The context's current launch choice (us_* are 0). */
+int voce_get_tuning(const voce_ctx *ctx, voce_tune_choice *out);
 
 /* ---- batches ---------------------------------------------------------- */
 
 typedef struct voce_batch voce_batch;
 
 /* n RVEs, one voce_setup each (grid sizes may differ; all must share one
- * precision).  tune = 1 autotunes each distinct grid size once.  Returns
- * 0 or the first member's error code. */
+ * precision).  tune = 1 times every FFT launch choice on each group of
+ * equal-size members as a whole (all of its blocks in one launch, as the
+ * iterations run) and keeps the fastest: throughput, where voce_autotune
+ * measures one RVE's latency.  Returns 0 or the first member's error code. */
 int voce_batch_create(const voce_setup *setups, int n, int tune, voce_batch **out);
 /* Advance every member by one strain increment.  Iterations of all members
  * run in the same kernels (block maps over the members), replayed from a

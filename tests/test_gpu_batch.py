@@ -38,7 +38,7 @@ def _run_alone(cases, n, precision, tune_choices=None):
     for i, cs in enumerate(cases):
         s = voce_amd.Solver.from_case(cs, precision)
         if tune_choices is not None:
-            s.autotune()
+            s.apply_tuning(tune_choices[i])
         r = [s.increment() for _ in range(n)]
         reps.append(r)
         fields.append({f: s.field(f) for f in FIELDS})
@@ -48,9 +48,11 @@ def _run_alone(cases, n, precision, tune_choices=None):
 
 def _run_batch(cases, n, precision, tune):
     b = voce_amd.Batch(cases, precision, tune=tune)
+    choices = [b.member(i).tuning() for i in range(len(cases))]
     reps = [b.increment() for _ in range(n)]
     fields = [{f: b.member(i).field(f) for f in FIELDS} for i in range(len(cases))]
     info = b.info()
+    info["choices"] = choices
     b.close()
     return reps, fields, info
 
@@ -80,7 +82,8 @@ def test_batch_tuned_equals_tuned_alone(tmp_path):
     cases = inputs.load_specs(str(tmp_path), _specs())
     n = 2
     rb, fb, info = _run_batch(cases, n, "fp64", tune=True)
-    ra, fa = _run_alone(cases, n, "fp64", tune_choices=True)   # same process cache: same choices
+    say("group-tuned choices", info["choices"])
+    ra, fa = _run_alone(cases, n, "fp64", tune_choices=info["choices"])   # the batch's choices, applied alone
     for i in range(len(cases)):
         for k in range(n):
             assert ra[i][k]["iter"] == rb[k][i]["iter"]

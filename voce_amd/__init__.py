@@ -89,7 +89,7 @@ EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_fi
            # include/voce_prof.h
            "voce_prof_enable", "voce_prof_read", "voce_ctx_info",
            # include/voce_batch.h
-           "voce_autotune", "voce_apply_tuning", "voce_batch_create", "voce_batch_increment",
+           "voce_autotune", "voce_apply_tuning", "voce_get_tuning", "voce_batch_create", "voce_batch_increment",
            "voce_batch_size", "voce_batch_member", "voce_batch_info", "voce_batch_destroy",
            "voce_set_velgrad", "voce_voronoi", "voce_load_case_voronoi",
            "voce_server_create", "voce_server_add_rve", "voce_server_submit", "voce_server_step",
@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH):
                                            C.POINTER(Case)]
     lib.voce_autotune.argtypes = [C.c_void_p, C.POINTER(TuneChoice)]
     lib.voce_apply_tuning.argtypes = [C.c_void_p, C.POINTER(TuneChoice)]
+    lib.voce_get_tuning.argtypes = [C.c_void_p, C.POINTER(TuneChoice)]
     lib.voce_set_velgrad.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     lib.voce_batch_create.argtypes = [C.POINTER(Setup), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     lib.voce_batch_increment.argtypes = [C.c_void_p, C.POINTER(Report)]
@@ -311,6 +312,20 @@ class Solver:
         if rc:
             raise VoceError(rc, "voce_autotune")
         return ch.as_dict()
+
+    def tuning(self) -> dict:
+        """The current FFT launch choice (plv, xv, tk, pl_r8, x_r8)."""
+        ch = TuneChoice()
+        rc = load_library().voce_get_tuning(self._h, C.byref(ch))
+        if rc:
+            raise VoceError(rc, "voce_get_tuning")
+        return {k: v for k, v in ch.as_dict().items() if not k.startswith("us_")}
+
+    def apply_tuning(self, choice: dict):
+        ch = TuneChoice(**{k: int(choice[k]) for k in ("plv", "xv", "tk", "pl_r8", "x_r8")})
+        rc = load_library().voce_apply_tuning(self._h, C.byref(ch))
+        if rc:
+            raise VoceError(rc, "voce_apply_tuning")
 
     def set_velgrad(self, L):
         a = (C.c_double * 9)(*[float(x) for x in np.asarray(L, dtype=np.float64).ravel()])

@@ -10,6 +10,7 @@
 // return at the top of every kernel, so the rest keep iterating.
 #include <hip/hip_runtime.h>
 
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,6 +28,8 @@ struct Group {
     int nx, ny, nz;
     std::vector<int> members;
     LaunchSet L;
+    int2 *dmap = nullptr;   // device block maps (batch-owned), reused when they fit
+    size_t dmap_cap = 0;
 };
 
 }  // namespace
@@ -85,9 +88,12 @@ static int build_group_maps(voce_batch *b, Group &g)
         if (d.plv != b->htab[g.members[0]].plv || d.xv != b->htab[g.members[0]].xv) return VOCE_ERR_ARG;
     }
     const size_t tot = pf.size() + p6.size() + p3.size() + x.size() + vox.size() + one.size();
-    int2 *dm = nullptr;
-    int rc = balloc(b, (void **)&dm, tot * sizeof(int2));
-    if (rc) return rc;
+    if (tot > g.dmap_cap) {
+        int rc = balloc(b, (void **)&g.dmap, tot * sizeof(int2));
+        if (rc) return rc;
+        g.dmap_cap = tot;
+    }
+    int2 *dm = g.dmap;
     std::vector<int2> all;
     all.reserve(tot);
     LaunchSet &L = g.L;
@@ -124,6 +130,129 @@ static int upload_tab(voce_batch *b)
     if (hipMemcpyAsync(b->dtab, b->htab.data(), b->n * sizeof(RveDev), hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return VOCE_ERR_DEVICE;
     return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : VOCE_ERR_DEVICE;
+}
+
+static int group_apply(voce_batch *b, Group &g, const voce_tune_choice &ch)
+{
+    int rc;
+    for (int i : g.members)
+        if ((rc = voce_apply_tuning(b->m[i], &ch))) return rc;
+    if ((rc = upload_tab(b))) return rc;
+    return build_group_maps(b, g);
+}
+
+// Throughput tuning of one group: every FFT launch choice is timed on the
+// whole group (all of its members' blocks in one launch, as an iteration
+// runs them).  A full GPU can favour other shapes than the latency of a
+// single RVE, which is what voce_autotune measures.  The probes overwrite
+// the spectra (scratch) and update E6, which is saved and restored.
+static int tune_group(voce_batch *b, Group &g)
+{
+    voce_ctx *c0 = b->m[g.members[0]];
+    const voce_tune_choice keep = {c0->d.plv, c0->d.xv, c0->d.tk, c0->d.pl_r8, c0->d.x_r8, 0.0, 0.0};
+    size_t tot = 0;
+    for (int i : g.members) tot += 6 * (size_t)b->m[i]->V;
+    double *save = nullptr;
+    if (hipMalloc((void **)&save, tot * sizeof(double)) != hipSuccess) return VOCE_ERR_NOMEM;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = VOCE_ERR_DEVICE;
+    const size_t act_off = offsetof(MacroState, active);
+    static const int flag_on = 1, flag_off = 0;
+    auto set_active = [&](const int *flag) {
+        for (int i : g.members)
+            if (hipMemcpyAsync((char *)(b->ms_all + i) + act_off, flag, sizeof(int), hipMemcpyHostToDevice, b->stream) !=
+                hipSuccess)
+                return VOCE_ERR_DEVICE;
+        return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : VOCE_ERR_DEVICE;
+    };
+    {
+        size_t off = 0;
+        for (int i : g.members) {
+            const size_t n6 = 6 * (size_t)b->m[i]->V;
+            if (!rc && hipMemcpyAsync(save + off, b->m[i]->d.E6, n6 * sizeof(double), hipMemcpyDeviceToDevice,
+                                      b->stream) != hipSuccess)
+                rc = VOCE_ERR_DEVICE;
+            off += n6;
+        }
+    }
+    if (!rc) rc = set_active(&flag_on);
+    auto probe = [&](int which, double *us) -> int {
+        const int reps = 3;
+        for (int i = 0; i < 2; ++i)
+            if (voce_launch_fft_probe(&g.L, b->stream, which)) return VOCE_ERR_DEVICE;
+        if (hipEventRecord(e0, b->stream) != hipSuccess) return VOCE_ERR_DEVICE;
+        for (int i = 0; i < reps; ++i)
+            if (voce_launch_fft_probe(&g.L, b->stream, which)) return VOCE_ERR_DEVICE;
+        if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return VOCE_ERR_DEVICE;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return VOCE_ERR_DEVICE;
+        *us = 1e3 * ms / reps;
+        return 0;
+    };
+    auto fits = [&](const voce_tune_choice &ch) {
+        RveDev d = c0->d;
+        d.tk = ch.tk;
+        return voce_plane_lds(&d, ch.plv) > 0 && voce_xgreen_lds(&d, ch.xv, ch.tk) > 0;
+    };
+    voce_tune_choice best = keep, cand;
+    double best_pl = 1e300, best_x = 1e300;
+    for (int plv = 0; plv < 4 && !rc; ++plv)
+        for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+            cand = keep;
+            cand.plv = plv;
+            cand.pl_r8 = r8;
+            if (!fits(cand)) continue;
+            if ((rc = group_apply(b, g, cand))) break;
+            double a = 0, bb = 0;
+            if ((rc = probe(0, &a)) || (rc = probe(2, &bb))) break;
+            if (a + bb < best_pl) {
+                best_pl = a + bb;
+                best.plv = plv;
+                best.pl_r8 = r8;
+            }
+        }
+    const int tks[5] = {1, 2, 4, 8, 16};
+    for (int xv = 0; xv < 3 && !rc; ++xv)
+        for (int ti = 0; ti < 5 && !rc; ++ti)
+            for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+                cand = keep;
+                cand.xv = xv;
+                cand.tk = tks[ti];
+                cand.x_r8 = r8;
+                if (!fits(cand)) continue;
+                if ((rc = group_apply(b, g, cand))) break;
+                double a = 0;
+                if ((rc = probe(1, &a))) break;
+                if (a < best_x) {
+                    best_x = a;
+                    best.xv = xv;
+                    best.tk = tks[ti];
+                    best.x_r8 = r8;
+                }
+            }
+    if (!rc && (best_pl >= 1e300 || best_x >= 1e300)) rc = VOCE_ERR_UNSUPPORTED_GRID;
+    best.us_plane = best_pl;
+    best.us_x = best_x;
+    // restore E6 and the flags whatever happened, then settle on a choice
+    {
+        size_t off = 0;
+        for (int i : g.members) {
+            const size_t n6 = 6 * (size_t)b->m[i]->V;
+            if (hipMemcpyAsync(b->m[i]->d.E6, save + off, n6 * sizeof(double), hipMemcpyDeviceToDevice, b->stream) !=
+                    hipSuccess && !rc)
+                rc = VOCE_ERR_DEVICE;
+            off += n6;
+        }
+    }
+    const int rc2 = set_active(&flag_off);
+    if (!rc) rc = rc2;
+    const int rc3 = group_apply(b, g, rc ? keep : best);
+    if (!rc) rc = rc3;
+    (void)hipFree(save);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
 }
 
 static void free_graphs(voce_batch *b)
@@ -216,17 +345,12 @@ extern "C" int voce_batch_create(const voce_setup *setups, int n, int tune, voce
         }
         b->groups[it->second].members.push_back(i);
     }
-    if (tune) {
-        for (Group &g : b->groups) {
-            voce_tune_choice ch;
-            if ((rc = voce_autotune(b->m[g.members[0]], &ch))) FAIL(rc);
-            for (size_t k = 1; k < g.members.size(); ++k)
-                if ((rc = voce_apply_tuning(b->m[g.members[k]], &ch))) FAIL(rc);
-        }
-    }
     if ((rc = upload_tab(b))) FAIL(rc);
     for (Group &g : b->groups)
         if ((rc = build_group_maps(b, g))) FAIL(rc);
+    if (tune)
+        for (Group &g : b->groups)
+            if ((rc = tune_group(b, g))) FAIL(rc);
     b->pred = b->iter_max_any < 8 ? b->iter_max_any : 8;
     if (b->pred < 1) b->pred = 1;
     (void)hipMemGetInfo(&free1, &total1);

@@ -110,8 +110,8 @@ struct RveDev {
     int fft_f32;                 // 1: FFT in FP32
     int uniform_phase;           // 1: one crystalline phase (its constants are uniform)
     int two_crss;                // 1: some phase keeps a separate backward CRSS
-    // FFT launch choices (tuned per grid size): plane-kernel variant plv and
-    // x-pass variant xv (0..2, see voce_kernels.hip), x-pass tile tk
+    // FFT launch choices (tuned per grid size): plane-kernel variant plv
+    // (0..3) and x-pass variant xv (0..2, see voce_kernels.hip), x-pass tile tk
     // ((y,kz) columns per block), radix-8 stages allowed in the plane and
     // x passes
     int plv, xv, tk, pl_r8, x_r8;

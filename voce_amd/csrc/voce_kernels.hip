@@ -1860,6 +1860,9 @@ constexpr int NT_FIN = 256;
 constexpr int NT_PL_T = 64, MAXEL_PL_T = 576;     // planes up to 32 x 32
 constexpr int NT_PL_S = 256, MAXEL_PL_S = 2112;   // planes up to 64 x 64
 constexpr int NT_PL_L = 512, MAXEL_PL_L = 8320;   // planes up to 128 x 128
+// 64 x 64 with five waves: the 33 half-spectrum columns of a 64 x 64 plane
+// are 264 radix-8 butterflies per stage, one per thread
+constexpr int NT_PL_M = 320, MAXEL_PL_M = 2112;
 constexpr int NT_X = 256, MAXEL_X = 3072;
 constexpr int NT_XB = 128, MAXEL_XB = 1536;
 constexpr int NT_XC = 64, MAXEL_XC = 768;
@@ -1870,6 +1873,7 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
     switch (L->plv) {
     case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_pfwd), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_pfwd); break;
     case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_pfwd), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 3: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>), dim3(L->n_pfwd), dim3(NT_PL_M), L->pl_lds, st, L->tab, L->map_pfwd); break;
     default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
@@ -1882,6 +1886,7 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
     switch (L->plv) {
     case 0: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_T, MAXEL_PL_T>), dim3(n), dim3(NT_PL_T), L->pl_lds, st, L->tab, map, mode); break;
     case 1: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(n), dim3(NT_PL_S), L->pl_lds, st, L->tab, map, mode); break;
+    case 3: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_M, MAXEL_PL_M>), dim3(n), dim3(NT_PL_M), L->pl_lds, st, L->tab, map, mode); break;
     default: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
     }
 }
@@ -1943,6 +1948,7 @@ hipError_t set_lds_limits()
     const void *fns[] = {(const void *)k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>, (const void *)k_plane_inv<T, NT_PL_T, MAXEL_PL_T>,
                          (const void *)k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>, (const void *)k_plane_inv<T, NT_PL_S, MAXEL_PL_S>,
                          (const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, (const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>,
+                         (const void *)k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>, (const void *)k_plane_inv<T, NT_PL_M, MAXEL_PL_M>,
                          (const void *)k_xgreen<T, NT_X, MAXEL_X>, (const void *)k_xgreen<T, NT_XB, MAXEL_XB>,
                          (const void *)k_xgreen<T, NT_XC, MAXEL_XC>};
     for (const void *f : fns) {
@@ -1977,7 +1983,7 @@ extern "C" int voce_grid_supported(int n)
 
 extern "C" int voce_vox_block(void) { return NT_VOX; }
 
-static const int PL_MAXEL[3] = {MAXEL_PL_T, MAXEL_PL_S, MAXEL_PL_L};
+static const int PL_MAXEL[4] = {MAXEL_PL_T, MAXEL_PL_S, MAXEL_PL_L, MAXEL_PL_M};
 static const int X_MAXEL[3] = {MAXEL_X, MAXEL_XB, MAXEL_XC};
 static const size_t LDS_MAX = 160 * 1024;
 
@@ -1999,7 +2005,7 @@ static const size_t PLANE_STATIC_LDS = sizeof(MacroState) + 16 * 8 + (NT_PL_L / 
 
 extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 {
-    if (plv < 0 || plv > 2 || r->ny * r->nzh > PL_MAXEL[plv]) return 0;
+    if (plv < 0 || plv > 3 || r->ny * r->nzh > PL_MAXEL[plv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
     const size_t b = (size_t)(r->ny * r->nzh + r->ny + r->nz) * cb;
     return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;

@@ -110,7 +110,7 @@ int measure(voce_ctx *c, voce_tune_choice *best)
     double best_pl = 1e300, best_x = 1e300;
     voce_tune_choice cand = keep, b = keep;
     // plane kernels: variant x radix set (x-pass choice held at its current value)
-    for (int plv = 0; plv < 3 && !rc; ++plv)
+    for (int plv = 0; plv < 4 && !rc; ++plv)
         for (int r8 = 1; r8 >= 0 && !rc; --r8) {
             cand = keep;
             cand.plv = plv;
@@ -175,6 +175,13 @@ extern "C" int voce_apply_tuning(voce_ctx *c, const voce_tune_choice *ch)
     if (!c || !ch) return VOCE_ERR_ARG;
     if (!choice_fits(c->d, *ch)) return VOCE_ERR_ARG;
     return apply(c, *ch);
+}
+
+extern "C" int voce_get_tuning(const voce_ctx *c, voce_tune_choice *out)
+{
+    if (!c || !out) return VOCE_ERR_ARG;
+    *out = voce_tune_choice{c->d.plv, c->d.xv, c->d.tk, c->d.pl_r8, c->d.x_r8, 0.0, 0.0};
+    return 0;
 }
 
 extern "C" int voce_autotune(voce_ctx *c, voce_tune_choice *out)
