@@ -98,6 +98,44 @@ def test_fp64_parity_gas_voxels(tmp_path):
     assert np.all(gf["sig"][:, :64] == 0.0)
 
 
+def test_fp64_parity_two_crystal_phases_generic_exponent(tmp_path):
+    # Type_phases 0 0: per-voxel phase constants (the multi-phase Newton
+    # kernel); phase II uses n = 10, the general integer power
+    m2 = inputs.Material(nsrs=10.0, tau0f=24.0, tau0b=24.0, tau1=5.0, theta0=300.0, theta1=40.0)
+    c = inputs.Case(dims=(8, 8, 8), n_grains=6, seed=31, n_steps=3, type_phases=(0, 0),
+                    elast2=(204600.0, 137700.0, 126200.0, 0.3), material2=m2, phase2_every=2)
+    compare(tmp_path, c, 3)
+
+
+def test_fp64_parity_creep(tmp_path):
+    # every component stress-controlled (CREEP_FLAG, init.c:736-740)
+    c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=35, n_steps=3, time_step=0.1,
+                    velgrad_flag=(0, 0, 0, 0, 0, 0), velgrad=(0.0,) * 6, stress_flag=(1, 1, 1, 1, 1, 1),
+                    stress=(25.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+    compare(tmp_path, c, 3)
+
+
+def test_fp64_parity_no_update_no_hardening(tmp_path):
+    c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=37, n_steps=3, update_flag=0, hard_flag=0)
+    compare(tmp_path, c, 3)
+
+
+def test_noninteger_exponent_aborts_like_reference(tmp_path):
+    # pow(r, n - 1) of a negative r is NaN for non-integer n
+    # (constitutive.c:40-41): the Newton matrix turns singular and the
+    # reference exits with 35; the library returns 35 from the same increment
+    c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=33, n_steps=1, material=inputs.Material(nsrs=12.5))
+    path, base = write(tmp_path, c)
+    import voce_amd
+    with pytest.raises(RuntimeError, match="code 35"):
+        oracle_run(path, base, 1)
+    s = voce_amd.Solver.from_case(voce_amd.load_case(path, base))
+    with pytest.raises(voce_amd.VoceError) as ei:
+        s.increment()
+    s.close()
+    assert ei.value.code == 35
+
+
 def test_deterministic_repeat(tmp_path):
     c = inputs.Case(dims=(8, 8, 8), n_grains=3, seed=2, n_steps=3)
     path, base = write(tmp_path, c)

@@ -369,7 +369,10 @@ static int setup_phase(const voce_phase &in, PhaseConst &pc, double C4[3][3][3][
         v_to_t(a5, p33, 5);    // as update_schmid expands it (kinematics.c:440-441)
         pc.p3[s][0] = p33[0][0]; pc.p3[s][1] = p33[1][1]; pc.p3[s][2] = p33[2][2];
         pc.p3[s][3] = p33[1][2]; pc.p3[s][4] = p33[0][2]; pc.p3[s][5] = p33[0][1];
-        for (int m = 0; m < 3; ++m) { pc.bn[s][m] = bn[m]; pc.bb[s][m] = bb[m]; }
+        for (int m = 0; m < 3; ++m) {
+            pc.bn[s][m] = pc.rn[s][m] = bn[m];
+            pc.bb[s][m] = pc.rb[s][m] = bb[m];
+        }
     }
     for (int i = 0; i < VOCE_NSYS; ++i)
         for (int j = 0; j < VOCE_NSYS; ++j) pc.hard[i][j] = in.hard[i][j];
@@ -402,6 +405,43 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     for (int p = 0; p < su->n_phases; ++p) {
         int rc = setup_phase(su->phase[p], pcs[p], Cph[p]);
         if (rc) return rc;
+    }
+    if (su->n_phases == 2 && !pcs[0].gas && !pcs[1].gas) {
+        // PlasticInit's normalisation loop runs over the mode index, not the
+        // phase (init.c:195-224): the second phase's call normalises the
+        // first phase's slip vectors once more and builds the second phase's
+        // Schmid tensors from them, while the second phase's own vectors stay
+        // as read, unnormalised, for update_orient (constitutive.c:202-203).
+        // A second phase with more systems than the first would read unset
+        // vectors there; it is refused.
+        PhaseConst &a = pcs[0], &b = pcs[1];
+        const voce_phase &ib = su->phase[1];
+        if (b.nsys > a.nsys) return VOCE_ERR_ARG;
+        for (int s = 0; s < b.nsys; ++s) {
+            double nn = 0, nb = 0, dot = 0, bn[3], bb[3];
+            for (int m = 0; m < 3; ++m) { nn += pow(a.bn[s][m], 2.0); nb += pow(a.bb[s][m], 2.0); }
+            nn = sqrt(nn);
+            nb = sqrt(nb);
+            for (int m = 0; m < 3; ++m) { bn[m] = a.bn[s][m] / nn; bb[m] = a.bb[s][m] / nb; }
+            for (int m = 0; m < 3; ++m) dot += bn[m] * bb[m];
+            if (dot > 1E-3) return VOCE_ERR_SLIP_NOT_ORTHOGONAL;
+            double a33[3][3], a5[5], p33[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) a33[i][j] = (bn[i] * bb[j] + bn[j] * bb[i]) / 2.0;
+            t_to_v(a33, a5, 5);
+            for (int i = 0; i < 5; ++i) b.px[s][i] = a5[i];
+            v_to_t(a5, p33, 5);
+            b.p3[s][0] = p33[0][0]; b.p3[s][1] = p33[1][1]; b.p3[s][2] = p33[2][2];
+            b.p3[s][3] = p33[1][2]; b.p3[s][4] = p33[0][2]; b.p3[s][5] = p33[0][1];
+            for (int m = 0; m < 3; ++m) {
+                a.rn[s][m] = bn[m];   // the first phase rotates its twice-normalised vectors
+                a.rb[s][m] = bb[m];
+                b.bn[s][m] = bn[m];
+                b.bb[s][m] = bb[m];
+                b.rn[s][m] = ib.bn[s][m];
+                b.rb[s][m] = ib.bb[s][m];
+            }
+        }
     }
     if (su->n_phases == 1) { pcs[1] = pcs[0]; pcs[1].gas = 1; pcs[1].nsys = 0; }
 

@@ -52,7 +52,8 @@ struct PhaseConst {
     double nsr, gam0, tau0f, tau0b, tau1, thet0, thet1;
     double px[12][5];      // crystal-frame Schmid vectors Schm_xt (init.c:217-224)
     double p3[12][6];      // the same as symmetric tensors: 00 11 22 12 02 01
-    double bn[12][3], bb[12][3];
+    double bn[12][3], bb[12][3];   // slip normal and direction whose product is px
+    double rn[12][3], rb[12][3];   // the slip vectors update_orient rotates (constitutive.c:202-203)
     double hard[12][12];
 };
 

@@ -1658,8 +1658,8 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
                             nsa[i] = 0.0;
 #pragma unroll
                             for (int j = 0; j < 3; ++j) {
-                                bsa[i] += Rm[i * 3 + j] * pc->bb[s][j];
-                                nsa[i] += Rm[i * 3 + j] * pc->bn[s][j];
+                                bsa[i] += Rm[i * 3 + j] * pc->rb[s][j];
+                                nsa[i] += Rm[i * 3 + j] * pc->rn[s][j];
                             }
                         }
 #pragma unroll

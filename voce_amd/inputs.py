@@ -139,6 +139,12 @@ class Case:
     hard_flag: int = 1
     print_control: tuple = (0, 1)
     tex_flag: int = 0
+    # phase II: gas (Type_phases 1) unless made crystalline with its own
+    # constants; grains whose id is a multiple of phase2_every get phase 2
+    type_phases: tuple = (0, 1)
+    elast2: tuple = (0.0, 0.0, 0.0, 0.0)
+    material2: Material = None
+    phase2_every: int = 0
 
     @property
     def nvox(self):
@@ -160,11 +166,11 @@ def write_namelist(path, c: Case, mesh_name="mesh.txt", sx_name="material.sx"):
         ("N_phases", "2"),
         ("CellDim", " ".join(map(str, c.dims))),
         ("initial_ms", mesh_name),
-        ("Type_phases", "0 1"),
+        ("Type_phases", " ".join(map(str, c.type_phases))),
         ("ElastConst_PhaseI", " ".join(_fmt(float(x)) for x in c.elast)),
         ("Slip_PhaseI", sx_name),
-        ("ElastConst_PhaseII", "0.0 0.0 0.0 0.0"),
-        ("Slip_PhaseII", "XXXXX"),
+        ("ElastConst_PhaseII", " ".join(_fmt(float(x)) for x in c.elast2)),
+        ("Slip_PhaseII", "material2.sx" if c.type_phases[1] == 0 else "XXXXX"),
         ("VelGrad_BC_Flag", " ".join(map(str, c.velgrad_flag))),
         ("VelGrad_BC", " ".join(_fmt(float(x)) for x in c.velgrad)),
         ("Stress_BC_Flag", " ".join(map(str, c.stress_flag))),
@@ -201,7 +207,12 @@ def write_case(directory, c: Case, grain=None, euler=None):
     if grain is None:
         grain, euler = voronoi_grains(c.dims, c.n_grains, c.seed)
     write_material(os.path.join(directory, "material.sx"), c.material)
-    write_mesh(os.path.join(directory, "mesh.txt"), c.dims, grain, euler)
+    if c.type_phases[1] == 0:
+        write_material(os.path.join(directory, "material2.sx"), c.material2 or c.material)
+    phase = None
+    if c.phase2_every > 0:
+        phase = np.where(np.asarray(grain) % c.phase2_every == 0, 2, 1).astype(np.int32)
+    write_mesh(os.path.join(directory, "mesh.txt"), c.dims, grain, euler, phase)
     p = os.path.join(directory, "input")
     write_namelist(p, c)
     return p
