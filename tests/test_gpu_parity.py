@@ -59,6 +59,13 @@ def test_fp64_parity_noncubic_radix3(tmp_path):
     compare(tmp_path, c, 3)
 
 
+@pytest.mark.parametrize("dims,grains", [((9, 6, 4), 4), ((3, 4, 6), 3), ((2, 2, 2), 2)])
+def test_fp64_parity_odd_and_tiny_grids(tmp_path, dims, grains):
+    # odd Nx (radix-3 x pass), the smallest grid the kernels take
+    c = inputs.Case(dims=dims, n_grains=grains, seed=41, n_steps=2)
+    compare(tmp_path, c, 2)
+
+
 def test_fp64_parity_plane_strain_all_velocity(tmp_path):
     c = inputs.Case(dims=(12, 12, 12), n_grains=6, seed=13, n_steps=3, velgrad_flag=(1, 1, 1, 1, 1, 1),
                     velgrad=(1.0e-3, 0.0, -1.0e-3, 0.0, 0.0, 0.0), stress_flag=(0, 0, 0, 0, 0, 0))
