@@ -91,7 +91,8 @@ typedef struct voce_server_opts {
     int max_rves;             /* ids that can be registered (table capacity) */
     int max_inflight;         /* increments iterating at once (0: memory-bound only) */
     size_t workspace_bytes;   /* workspace pool (0: an eighth of the free device memory, at most 16 GiB) */
-    int tune;                 /* autotune each grid size when first registered */
+    int tune;                 /* tune each grid size when first registered (timed for
+                                 throughput on a temporary batch of copies of that RVE) */
     int chunk;                /* most FFT iterations launched between polls (0: 16); a round
                                  launches up to the nearest predicted end of an increment in
                                  flight (each id's previous iteration count), at least 2 */

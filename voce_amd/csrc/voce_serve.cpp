@@ -312,11 +312,25 @@ extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
         auto it = s->tuned.find(key);
         voce_tune_choice ch;
         if (it == s->tuned.end()) {
-            // the tuner runs the FFT kernels: lend the context a workspace
-            if (!take_ws(s, id, c->d)) return VOCE_ERR_NOMEM;
-            if ((rc = voce_sync_tab(c))) return rc;
-            rc = voce_autotune(c, &ch);
-            drop_ws(s, id, c->d);
+            // A round launches the kernels over many RVEs at once, so the
+            // choice is timed for throughput: on a temporary batch of copies
+            // of this RVE, about 2 M voxels in all (at most 64 copies).
+            const long per = c->V > 0 ? c->V : 1;
+            int k = (int)((2L << 20) / per);
+            k = k < 1 ? 1 : (k > 64 ? 64 : k);
+            std::vector<voce_setup> cps((size_t)k, cp);
+            voce_batch *tb = nullptr;
+            rc = voce_batch_create(cps.data(), k, 1, &tb);
+            if (rc == 0) {
+                rc = voce_get_tuning(voce_batch_member(tb, 0), &ch);
+                voce_batch_destroy(tb);
+            } else {
+                // no room for the copies: one RVE's latency, on a lent workspace
+                if (!take_ws(s, id, c->d)) return VOCE_ERR_NOMEM;
+                if ((rc = voce_sync_tab(c))) return rc;
+                rc = voce_autotune(c, &ch);
+                drop_ws(s, id, c->d);
+            }
             if (rc) return rc;
             s->tuned[key] = ch;
         } else {
