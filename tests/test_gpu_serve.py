@@ -85,7 +85,7 @@ def test_served_equals_context(tmp_path):
         s.submit(rid, L, False, tag)
     served = sorted(s.drain(), key=lambda r: r["tag"])
     c = voce_amd.Solver.from_case(cases[1], "mixed")
-    c.autotune()
+    c.apply_tuning(s.rve(1).tuning())   # the server's launch choice: the same kernels
     for (rid, L, _, tag), r in zip(reqs, served):
         c.set_velgrad(L)
         _same(c.increment() | {"tag": tag}, r)

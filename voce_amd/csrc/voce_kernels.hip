@@ -167,6 +167,34 @@ template <bool INV, typename C> struct Bfly<8, INV, C> {
     }
 };
 
+// Index arithmetic of the FFT kernels.  Every index there is below 2^22,
+// so products use the full-rate 24-bit multiply, and a / d comes from a
+// float reciprocal with one correction step ((float)a is exact and a * (1/d)
+// is within 1/2 of a/d, so the truncated quotient is off by at most one)
+// instead of the integer division sequence with its quarter-rate multiplies.
+__device__ __forceinline__ int mul24(int a, int b) { return (int)__umul24((unsigned)a, (unsigned)b); }
+
+struct FastDiv {
+    int d;
+    float r;
+    __device__ __forceinline__ explicit FastDiv(int d_) : d(d_), r(1.0f / (float)d_) {}
+    // a / d, with a % d in rem (0 <= a < 2^22)
+    __device__ __forceinline__ int div(int a, int &rem) const
+    {
+        int q = (int)((float)a * r);
+        int m = a - mul24(q, d);
+        if (m < 0) {
+            --q;
+            m += d;
+        } else if (m >= d) {
+            ++q;
+            m -= d;
+        }
+        rem = m;
+        return q;
+    }
+};
+
 // ------------------------------------------------------------------
 // One Stockham stage over `nlines` lines of length N held in LDS.
 // Element q of line l lives at s[l*lstride + q*estride].  All loads of a
@@ -180,19 +208,21 @@ __device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int l
     constexpr int BPT = (MAXEL / R + NT - 1) / NT;
     const int nbl = N / R;
     const int nb = nlines * nbl;
+    // all index products stay below 2^24: 24-bit multiplies
+    const FastDiv split(LF ? nlines : nbl), byns(Ns);
     C v[BPT][R];
     int base[BPT], jj[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
         const int b = (int)threadIdx.x + i * NT;
         if (b < nb) {
-            int l, j;
-            if (LF) { j = b / nlines; l = b - j * nlines; }
-            else { l = b / nbl; j = b - l * nbl; }
-            base[i] = l * lstride;
+            int l, j, m;
+            if (LF) { j = split.div(b, m); l = m; }
+            else { l = split.div(b, m); j = m; }
+            base[i] = mul24(l, lstride);
             jj[i] = j;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = s[base[i] + (j + r * nbl) * estride];
+            for (int r = 0; r < R; ++r) v[i][r] = s[base[i] + mul24(j + r * nbl, estride)];
         }
     }
     __syncthreads();
@@ -202,17 +232,18 @@ __device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int l
         const int b = (int)threadIdx.x + i * NT;
         if (b < nb) {
             const int j = jj[i];
-            const int k = j % Ns;
+            int k;
+            const int jd = byns.div(j, k);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                C w = tw[r * k * twstep];
+                C w = tw[mul24(r * k, twstep)];
                 if (INV) w.y = -w.y;
                 v[i][r] = cmul(v[i][r], w);
             }
             Bfly<R, INV, C>::run(v[i]);
-            const int d = (j / Ns) * Ns * R + k;
+            const int d = mul24(jd, Ns * R) + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[base[i] + (d + r * Ns) * estride] = v[i][r];
+            for (int r = 0; r < R; ++r) s[base[i] + mul24(d + r * Ns, estride)] = v[i][r];
         }
     }
     __syncthreads();
@@ -302,8 +333,10 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
     for (int i = tid; i < nz; i += NT) twz[i] = gtz[i];
     const gptr<double> src = as_global(r.S6 + (long)c * r.V + (long)x * ny * nz);
     T *sr = (T *)s;
+    const FastDiv bynz(nz), bynzh(nzh);
     grouped<NT, 16, double>(ny * nz, [&](int e) { return src[e]; }, [&](int e, double v) {
-        const int y = e / nz, z = e - y * nz;
+        int z;
+        const int y = bynz.div(e, z);
         sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)] = (T)v;
     });
     __syncthreads();
@@ -317,7 +350,8 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
             if (it < nit) {
-                const int p = it / nzh, k = it - p * nzh;
+                int k;
+                const int p = bynzh.div(it, k);
                 const C z0 = s[p * 2 * nzh + k];
                 const int km = (k == 0) ? 0 : nz - k;
                 const C z1 = s[p * 2 * nzh + km];
@@ -330,7 +364,8 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
             if (it < nit) {
-                const int p = it / nzh, k = it - p * nzh;
+                int k;
+                const int p = bynzh.div(it, k);
                 s[(2 * p) * nzh + k] = A[i];
                 s[(2 * p + 1) * nzh + k] = B[i];
             }
@@ -361,6 +396,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     C *twy = s + ny * nzh;
     C *twz = twy + ny;
     const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
+    const FastDiv bynz(nz);
     const C *gty = (const C *)r.tw_y;
     const C *gtz = (const C *)r.tw_z;
     for (int i = tid; i < ny; i += NT) twy[i] = gty[i];
@@ -381,7 +417,8 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
             if (it < nit) {
-                const int p = it / nz, k = it - p * nz;
+                int k;
+                const int p = bynz.div(it, k);
                 C A, B;
                 if (k <= nz / 2) {
                     A = s[(2 * p) * nzh + k];
@@ -401,7 +438,8 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
             if (it < nit) {
-                const int p = it / nz, k = it - p * nz;
+                int k;
+                const int p = bynz.div(it, k);
                 s[p * 2 * nzh + k] = Z[i];
             }
         }
@@ -417,14 +455,16 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         const double sh = creep_set ? 0.0 : ms->shift6[c];
         const double cr = ms->creep_sym6[c];
         grouped<NT, 16, double>(ny * nz, [&](int e) { return creep_set ? 0.0 : E[e]; }, [&](int e, double ev) {
-            const int y = e / nz, z = e - y * nz;
+            int z;
+            const int y = bynz.div(e, z);
             const double val = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
             E[e] = creep_set ? cr + (sh + val) : ev + (sh + val);
         });
     } else {
         double *W = r.WF + off;
         for (int e = tid; e < ny * nz; e += NT) {
-            const int y = e / nz, z = e - y * nz;
+            int z;
+            const int y = bynz.div(e, z);
             W[e] = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
         }
     }
@@ -621,10 +661,12 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     if (mode == 0) fft_lines<C, NT, MAXEL, false, false>(s, 6 * tk, nx, NXP, 1, twx, r.x_r8);
     const T invV = (T)(1.0 / (double)r.V);
     if (mode == 0) {
+        const FastDiv bynx(nx), bynzh(nzh);
         for (int e = tid; e < nx * ncol; e += NT) {
-            const int t = e / nx, x = e - t * nx;
+            int x, kz;
+            const int t = bynx.div(e, x);
             const int col = col0 + t;
-            const int ky = col / nzh, kz = col - ky * nzh;
+            const int ky = bynzh.div(col, kz);
             C a[6];
 #pragma unroll
             for (int cc = 0; cc < 6; ++cc) a[cc] = s[(cc * tk + t) * NXP + x];

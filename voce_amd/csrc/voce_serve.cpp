@@ -28,6 +28,7 @@
 
 #include <deque>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -290,6 +291,10 @@ static void drop_ws(voce_server *s, int row, RveDev &d)
     d.part = nullptr;
 }
 
+// throughput-tuned launch choices per (grid, precision), kept for the process
+static std::mutex g_srv_tune_mu;
+static std::map<std::tuple<int, int, int, int>, voce_tune_choice> g_srv_tune;
+
 extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
 {
     if (!s || !su || id < 0 || id >= s->o.max_rves) return VOCE_ERR_ARG;
@@ -311,10 +316,18 @@ extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
         const auto key = std::make_tuple(c->dims[0], c->dims[1], c->dims[2]);
         auto it = s->tuned.find(key);
         voce_tune_choice ch;
+        const auto gkey = std::make_tuple(c->dims[0], c->dims[1], c->dims[2], s->o.precision);
+        if (it == s->tuned.end()) {
+            std::lock_guard<std::mutex> lk(g_srv_tune_mu);
+            auto git = g_srv_tune.find(gkey);
+            if (git != g_srv_tune.end()) it = s->tuned.emplace(key, git->second).first;
+        }
         if (it == s->tuned.end()) {
             // A round launches the kernels over many RVEs at once, so the
             // choice is timed for throughput: on a temporary batch of copies
-            // of this RVE, about 2 M voxels in all (at most 64 copies).
+            // of this RVE, about 2 M voxels in all (at most 64 copies).  The
+            // choice is kept for the process, so servers in one process run
+            // a grid size with the same kernels (and the same rounding).
             const long per = c->V > 0 ? c->V : 1;
             int k = (int)((2L << 20) / per);
             k = k < 1 ? 1 : (k > 64 ? 64 : k);
@@ -333,6 +346,8 @@ extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
             }
             if (rc) return rc;
             s->tuned[key] = ch;
+            std::lock_guard<std::mutex> lk(g_srv_tune_mu);
+            g_srv_tune.emplace(gkey, ch);
         } else {
             ch = it->second;
         }
