@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -146,7 +147,32 @@ static int group_apply(voce_batch *b, Group &g, const voce_tune_choice &ch)
 // runs them).  A full GPU can favour other shapes than the latency of a
 // single RVE, which is what voce_autotune measures.  The probes overwrite
 // the spectra (scratch) and update E6, which is saved and restored.
+// group-tuned choices per (grid, precision, members), kept for the process so
+// that equal batches run the same kernels (and round the same way)
+static std::mutex g_grp_tune_mu;
+static std::map<std::tuple<int, int, int, int, int>, voce_tune_choice> g_grp_tune;
+
+static int tune_group_timed(voce_batch *b, Group &g);
+
 static int tune_group(voce_batch *b, Group &g)
+{
+    const auto key = std::make_tuple(g.nx, g.ny, g.nz, b->precision, (int)g.members.size());
+    {
+        std::lock_guard<std::mutex> lk(g_grp_tune_mu);
+        auto it = g_grp_tune.find(key);
+        if (it != g_grp_tune.end()) return group_apply(b, g, it->second);
+    }
+    const int rc = tune_group_timed(b, g);
+    if (rc) return rc;
+    voce_tune_choice ch;
+    if (voce_get_tuning(b->m[g.members[0]], &ch) == 0) {
+        std::lock_guard<std::mutex> lk(g_grp_tune_mu);
+        g_grp_tune.emplace(key, ch);
+    }
+    return 0;
+}
+
+static int tune_group_timed(voce_batch *b, Group &g)
 {
     voce_ctx *c0 = b->m[g.members[0]];
     const voce_tune_choice keep = {c0->d.plv, c0->d.xv, c0->d.tk, c0->d.pl_r8, c0->d.x_r8, 0.0, 0.0};
