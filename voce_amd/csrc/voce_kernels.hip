@@ -894,17 +894,6 @@ __device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[
     }
 }
 
-// Same, with the basis rotation rebuilt from R (keeps 9 values live
-// across the Newton loop instead of 25).
-__device__ __forceinline__ void flow_rule(const double sig[6], const double R[9], const PhaseConst *pc, int nsys,
-                                          const double cf[12], const double cb[12], double ed[5], double D[15],
-                                          double *gd, double *tx)
-{
-    double Q[5][5];
-    basis_rotation(R, Q);
-    flow_rule_q(sig, Q, pc, nsys, cf, cb, ed, D, gd, tx);
-}
-
 __device__ __forceinline__ double Dget(const double D[15], int i, int j)
 {
     const int a = i < j ? i : j, b = i < j ? j : i;
@@ -1196,32 +1185,6 @@ __device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6
     return bad ? 35 : 0;
 }
 
-// B-basis (b1..b5) components of R P R^T for a symmetric crystal-frame
-// tensor P given as (P00, P11, P22, P12, P02, P01)
-__device__ __forceinline__ void rotate_sym_b5(const double R[9], const double *p, double out[5])
-{
-    // M = R P (3x3), then only the needed entries of M R^T
-    double M[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const double r0 = R[i * 3 + 0], r1 = R[i * 3 + 1], r2 = R[i * 3 + 2];
-        M[i * 3 + 0] = r0 * p[0] + r1 * p[5] + r2 * p[4];
-        M[i * 3 + 1] = r0 * p[5] + r1 * p[1] + r2 * p[3];
-        M[i * 3 + 2] = r0 * p[4] + r1 * p[3] + r2 * p[2];
-    }
-    const double q00 = M[0] * R[0] + M[1] * R[1] + M[2] * R[2];
-    const double q11 = M[3] * R[3] + M[4] * R[4] + M[5] * R[5];
-    const double q22 = M[6] * R[6] + M[7] * R[7] + M[8] * R[8];
-    const double q12 = M[3] * R[6] + M[4] * R[7] + M[5] * R[8];
-    const double q02 = M[0] * R[6] + M[1] * R[7] + M[2] * R[8];
-    const double q01 = M[0] * R[3] + M[1] * R[4] + M[2] * R[5];
-    out[0] = RSQ2 * (q11 - q00);
-    out[1] = RSQ6 * (2. * q22 - q00 - q11);
-    out[2] = 2. * RSQ2 * q12;
-    out[3] = 2. * RSQ2 * q02;
-    out[4] = 2. * RSQ2 * q01;
-}
-
 template <int NT, bool UNI, bool TWO>
 __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
@@ -1252,7 +1215,6 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
             const int nsys = pc->nsys;
             const cptr<MacroState> msc = as_const(ms);
             const double dt = msc->dt;
-            const cptr<double> C0 = msc->C066;
             const gptr<double> tab = as_global(r.SLOC + (long)ori * VOCE_OTAB);   // S_local, K_c, initial TranMat
             // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
             // it would form per evaluation)
