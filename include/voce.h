@@ -134,8 +134,17 @@ enum voce_field {
     VOCE_FIELD_RHO_RSS = 15, /* 1 (evolution.c:523) */
     VOCE_FIELD_RHO_CRSS = 16,/* 1 (evolution.c:524) */
     VOCE_FIELD_VELGRAD = 17, /* 9: skew part only (the part the solver uses) */
-    VOCE_FIELD_SCHMID = 18,  /* 60: Schm_gr[.][s][0..4] */
-    VOCE_FIELD_COUNT = 19
+    VOCE_FIELD_SCHMID = 18,  /* 60: Schm_gr[.][s][0..4] of the current TranMat
+                                (update_schmid, kinematics.c:426-459): the
+                                reference's Schm_gr after the next increment's
+                                refresh (evolution.c:320-322) */
+    VOCE_FIELD_WORK_E = 19,  /* 1: 0.5 Sig : els33, els in the order above read as
+                                B-basis coefficients (WriteWorkEMPI, io.c:231-278) */
+    VOCE_FIELD_NEW_POSITION = 20, /* 3: -fluctuation[.][i][2], i = 0..2: what the
+                                displacement block writes as new_position
+                                (evolution.c:532-560); tracked while diagnostics
+                                are on, 0 otherwise */
+    VOCE_FIELD_COUNT = 21
 };
 
 typedef struct voce_ctx voce_ctx;

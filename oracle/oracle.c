@@ -1557,6 +1557,41 @@ int orc_get_field(orc_job *J, int field, double *out)
     case 19: /* Newton iterations of the last UpdateStress (diagnostic) */
         memcpy(out, J->nr_iters, sizeof(double) * V);
         return 1;
+    case 20: /* work_e, WriteWorkEMPI io.c:231-278: the Voigt-ordered els through chg_basis opt 1 */
+        for (p = 0; p < V; p++) {
+            double els[6], w = 0.0;
+            t2 els33;
+            int mi, mj, i;
+            for (i = 0; i < 6; i++) {
+                if (i < 3) els[i] = J->DisGrad[p][i][i] - J->Eps[p][i][i];
+                else if (i == 3) els[i] = (J->DisGrad[p][1][2] + J->DisGrad[p][2][1]) / 2.0 - J->Eps[p][1][2];
+                else if (i == 4) els[i] = (J->DisGrad[p][0][2] + J->DisGrad[p][2][0]) / 2.0 - J->Eps[p][0][2];
+                else els[i] = (J->DisGrad[p][0][1] + J->DisGrad[p][1][0]) / 2.0 - J->Eps[p][0][1];
+            }
+            chg_basis(els, els33, NULL, NULL, 1);
+            for (mi = 0; mi < 3; mi++)
+                for (mj = 0; mj < 3; mj++) w += 0.5 * J->Sig[p][mi][mj] * els33[mi][mj];
+            out[p] = w;
+        }
+        return 1;
+    case 21: { /* new_position, evolution.c:532-560: fft_data holds the forward
+                  transform of fluctuation[.][mi][2] (the last mj) when the
+                  Green step's store lands past the array, so the inverse gives
+                  back V * fluctuation[.][mi][2] */
+        cpx *d = (cpx *)malloc(sizeof(cpx) * V);
+        int mi;
+        for (mi = 0; mi < 3; mi++) {
+            for (p = 0; p < V; p++) {
+                d[p].re = J->fluctuation[p][mi][2];
+                d[p].im = 0.0;
+            }
+            fft3d_cpx(d, J->Nx, J->Ny, J->Nz, -1);
+            fft3d_cpx(d, J->Nx, J->Ny, J->Nz, +1);
+            for (p = 0; p < V; p++) out[mi * V + p] = -d[p].re / (double)V;
+        }
+        free(d);
+        return 3;
+    }
     case F_SCHMID:
         for (c = 0; c < 60; c++)
             for (p = 0; p < V; p++) out[c * V + p] = J->Schm_gr[p][c / 5][c % 5];
@@ -1654,6 +1689,17 @@ int orc_write_outputs(orc_job *J, const char *dir, int step, int tex_flag)
         snprintf(name, sizeof(name), "work_p_S%06d.iout", step); write_doubles(dir, name, J->work_p, V);
         snprintf(name, sizeof(name), "SVM_S%04d.iout", step); write_doubles(dir, name, J->SVM, V);
         snprintf(name, sizeof(name), "EVM_S%04d.iout", step); write_doubles(dir, name, J->EVM, V);
+        /* new_position, io.c:805-849: 24-byte {3 double} records */
+        {
+            double *np3 = (double *)malloc(sizeof(double) * 3 * V), *rec3 = (double *)malloc(sizeof(double) * 3 * V);
+            orc_get_field(J, 21, np3);
+            for (p = 0; p < V; p++)
+                for (c = 0; c < 3; c++) rec3[3 * p + c] = np3[c * V + p];
+            snprintf(name, sizeof(name), "new_position_S%04d.iout", step);
+            write_doubles(dir, name, rec3, 3 * V);
+            free(np3);
+            free(rec3);
+        }
     }
     free(buf);
     return 0;

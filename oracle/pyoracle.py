@@ -70,11 +70,11 @@ def ptr(a: np.ndarray):
 
 
 FIELD_NCOMP = {0: 6, 1: 6, 2: 6, 3: 6, 4: 9, 5: 12, 6: 12, 7: 12, 8: 12, 9: 12, 10: 9, 11: 1, 12: 1,
-               13: 1, 14: 1, 15: 1, 16: 1, 17: 9, 18: 60, 19: 1}
+               13: 1, 14: 1, 15: 1, 16: 1, 17: 9, 18: 60, 19: 1, 20: 1, 21: 3}
 FIELD_IDS = {"sig": 0, "eps": 1, "edot": 2, "els": 3, "disgrad": 4, "crss": 5, "crss_bwd": 6,
              "gamtot": 7, "gamdot": 8, "taux": 9, "tranmat": 10, "gamacum": 11, "svm": 12, "evm": 13,
              "work_p": 14, "rho_rss": 15, "rho_crss": 16, "velgrad": 17, "schmid": 18,
-             "nr_iters": 19}
+             "nr_iters": 19, "work_e": 20, "new_position": 21}
 
 
 def report_dict(r: OrcReport) -> dict:

@@ -59,7 +59,12 @@ def test_driver_outputs_match_oracle(tmp_path):
     for s in range(1, c.n_steps + 1):
         j.increment()
         j.write_outputs(str(b), s, 1)
+    work_e = j.field("work_e").ravel()
     j.close()
+    # work_e is written only at the end (evolution.c:603)
+    we = np.frombuffer((a / "work_e_final_S000000.iout").read_bytes(), dtype=np.float64)
+    assert we.shape == work_e.shape
+    assert np.abs(we - work_e).max() <= 1e-7 * max(np.abs(work_e).max(), 1e-300)
     # vm.out: %e (7 significant digits); err.out rows hold differences of nearby stresses
     _close_rows(_rows(a / "vm.out"), _rows(b / "vm.out"), 2e-6)
     _close_rows(_rows(a / "err.out"), _rows(b / "err.out"), 1e-4)

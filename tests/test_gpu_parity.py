@@ -11,12 +11,12 @@ from gpu_helpers import gpu_run, oracle_run, rel, say, write
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("sig", "eps", "edot", "crss", "tranmat", "gamtot", "gamacum", "disgrad", "svm", "evm", "taux",
-          "gamdot", "work_p", "rho_rss", "rho_crss")
+          "gamdot", "work_p", "rho_rss", "rho_crss", "work_e", "new_position")
 
 # per-voxel tolerances, relative to the field's largest magnitude
 FIELD_TOL = {"sig": 1e-8, "eps": 1e-8, "edot": 1e-7, "crss": 1e-10, "tranmat": 1e-10, "gamtot": 1e-7,
              "gamacum": 1e-8, "disgrad": 1e-9, "svm": 1e-8, "evm": 1e-8, "taux": 1e-7, "gamdot": 1e-7,
-             "work_p": 1e-7, "rho_rss": 1e-8, "rho_crss": 1e-10}
+             "work_p": 1e-7, "rho_rss": 1e-8, "rho_crss": 1e-10, "work_e": 1e-8, "new_position": 1e-8}
 
 
 def compare(tmp_path, case, n, grain=None, euler=None, fields=FIELDS, macro_tol=1e-8):
@@ -141,6 +141,17 @@ def test_noninteger_exponent_aborts_like_reference(tmp_path):
         s.increment()
     s.close()
     assert ei.value.code == 35
+
+
+def test_schmid_field_is_the_next_increments(tmp_path):
+    # the Schmid field is update_schmid of the current TranMat, i.e. the
+    # reference's Schm_gr once its next increment has refreshed it
+    # (evolution.c:320-322, kinematics.c:426-459)
+    c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=43, n_steps=3)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, fields=("schmid",))
+    o, of = oracle_run(path, base, 3, fields=("schmid",))
+    assert rel(gf["schmid"], of["schmid"]) < 1e-10
 
 
 def test_deterministic_repeat(tmp_path):

@@ -30,7 +30,8 @@ FP64, MIXED = 0, 1
 FIELDS = {
     "sig": 0, "eps": 1, "edot": 2, "els": 3, "disgrad": 4, "crss": 5, "crss_bwd": 6,
     "gamtot": 7, "gamdot": 8, "taux": 9, "tranmat": 10, "gamacum": 11, "svm": 12, "evm": 13,
-    "work_p": 14, "rho_rss": 15, "rho_crss": 16, "velgrad": 17,
+    "work_p": 14, "rho_rss": 15, "rho_crss": 16, "velgrad": 17, "schmid": 18, "work_e": 19,
+    "new_position": 20,
 }
 
 

@@ -280,7 +280,9 @@ extern "C" int voce_field_ncomp(int f)
     case VOCE_FIELD_CRSS: case VOCE_FIELD_CRSS_BWD: case VOCE_FIELD_GAMTOT: case VOCE_FIELD_GAMDOT:
     case VOCE_FIELD_TAUX: return 12;
     case VOCE_FIELD_GAMACUM: case VOCE_FIELD_SVM: case VOCE_FIELD_EVM: case VOCE_FIELD_WORK_P:
-    case VOCE_FIELD_RHO_RSS: case VOCE_FIELD_RHO_CRSS: return 1;
+    case VOCE_FIELD_RHO_RSS: case VOCE_FIELD_RHO_CRSS: case VOCE_FIELD_WORK_E: return 1;
+    case VOCE_FIELD_NEW_POSITION: return 3;
+    case VOCE_FIELD_SCHMID: return 60;
     }
     return -1;
 }
@@ -785,7 +787,7 @@ static int ensure_diag(voce_ctx *c)
     const size_t db = sizeof(double);
     struct { double **dst; size_t n; } want[] = {{&d.GAMDOT, 12 * V * db}, {&d.TAUX, 12 * V * db}, {&d.SVM, V * db},
                                                 {&d.EVM, V * db}, {&d.WORKP, V * db}, {&d.RHOR, V * db},
-                                                {&d.RHOC, V * db}};
+                                                {&d.RHOC, V * db}, {&d.FLU, 6 * V * db}};
     for (auto &w : want) {
         if (voce_dev_alloc(c, &p, w.n)) return VOCE_ERR_NOMEM;
         if (hipMemsetAsync(p, 0, w.n, c->stream) != hipSuccess) return VOCE_ERR_DEVICE;
@@ -996,6 +998,85 @@ extern "C" int voce_get_field(voce_ctx *c, int field, double *host, size_t bytes
         }
         return 0;
     }
+    case VOCE_FIELD_WORK_E: {
+        // WriteWorkEMPI (io.c:231-278): els in Voigt order (11,22,33,23,13,12)
+        // goes through chg_basis(opt 1) as if it were a B-basis vector
+        a.resize(6 * V);
+        b.resize(6 * V);
+        if ((rc = voce_get_field(c, VOCE_FIELD_SIG, a.data(), 6 * V * db))) return rc;
+        if ((rc = voce_get_field(c, VOCE_FIELD_ELS, b.data(), 6 * V * db))) return rc;
+        for (long v = 0; v < V; ++v) {
+            double e6[6], T[3][3], S[3][3];
+            for (int i = 0; i < 6; ++i) e6[i] = b[i * V + v];
+            v_to_t(e6, T, 6);
+            const double s6[6] = {a[v], a[V + v], a[2 * V + v], a[3 * V + v], a[4 * V + v], a[5 * V + v]};
+            S[0][0] = s6[0]; S[1][1] = s6[1]; S[2][2] = s6[2];
+            S[1][2] = S[2][1] = s6[3];
+            S[0][2] = S[2][0] = s6[4];
+            S[0][1] = S[1][0] = s6[5];
+            double w = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) w += 0.5 * S[i][j] * T[i][j];
+            host[v] = w;
+        }
+        return 0;
+    }
+    case VOCE_FIELD_NEW_POSITION: {
+        // evolution.c:532-560 transforms fluctuation[.][i][2] forward and back
+        // (its Green step writes out of bounds and is lost): the output is
+        // -fluctuation[.][i][2]
+        if (!d.FLU) {
+            memset(host, 0, 3 * (size_t)V * db);
+            return 0;
+        }
+        a.resize(6 * V);
+        if ((rc = d2h(d.FLU, a.data(), 6 * V * db))) return rc;
+        for (long v = 0; v < V; ++v) {
+            const double x[6] = {0.0, a[v], a[V + v], a[2 * V + v], 0.0, a[3 * V + v]};
+            double T[3][3];
+            v_to_t(x, T, 6);
+            const double w0 = a[4 * V + v], w1 = a[5 * V + v];
+            host[v] = -(T[0][2] + w1);
+            host[V + v] = -(T[1][2] - w0);
+            host[2 * V + v] = -T[2][2];
+        }
+        return 0;
+    }
+    case VOCE_FIELD_SCHMID: {
+        // update_schmid (kinematics.c:426-459): Schm_xt as a tensor, rotated
+        // by TranMat, back to 5 components
+        a.resize(9 * V);
+        if ((rc = d2h(d.R, a.data(), 9 * V * db))) return rc;
+        std::vector<uint8_t> ph(V);
+        if ((rc = d2h(d.PH, ph.data(), V))) return rc;
+        PhaseConst pcs[2];
+        if ((rc = d2h(d.phase, pcs, sizeof(pcs)))) return rc;
+        for (long v = 0; v < V; ++v) {
+            const PhaseConst &pc = pcs[ph[v] >= 1 && ph[v] <= 2 ? ph[v] - 1 : 0];
+            double R[3][3];
+            for (int i = 0; i < 9; ++i) R[i / 3][i % 3] = a[i * V + v];
+            for (int sy = 0; sy < 12; ++sy) {
+                double out[5] = {0, 0, 0, 0, 0};
+                if (sy < pc.nsys && !pc.gas) {
+                    double P[3][3], M[3][3], Q[3][3];
+                    v_to_t(pc.px[sy], P, 5);
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            M[i][j] = 0.0;
+                            for (int k = 0; k < 3; ++k) M[i][j] += R[i][k] * P[k][j];
+                        }
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            Q[i][j] = 0.0;
+                            for (int k = 0; k < 3; ++k) Q[i][j] += M[i][k] * R[j][k];
+                        }
+                    t_to_v(Q, out, 5);
+                }
+                for (int k = 0; k < 5; ++k) host[(sy * 5 + k) * V + v] = out[k];
+            }
+        }
+        return 0;
+    }
     case VOCE_FIELD_CRSS: return d2h(d.CRSS, host, 12 * V * db);
     case VOCE_FIELD_CRSS_BWD: {
         if (c->two_crss_any) return d2h(d.CRSSB, host, 12 * V * db);
@@ -1092,6 +1173,21 @@ extern "C" int voce_write_fields(voce_ctx *c, const char *dir, int step, int tex
     for (auto &s : one) {
         if ((rc = voce_get_field(c, s.field, buf.data(), buf.size() * sizeof(double)))) return rc;
         snprintf(name, sizeof(name), s.wide ? "%s_S%06d.iout" : "%s_S%04d.iout", fin ? s.pre_fin : s.pre, step);
+        if ((rc = write_raw(dir, name, buf.data(), V * sizeof(double)))) return rc;
+    }
+    // new_position: 24-byte {3 double} records (io.c:805-849)
+    if ((rc = voce_get_field(c, VOCE_FIELD_NEW_POSITION, buf.data(), buf.size() * sizeof(double)))) return rc;
+    {
+        std::vector<double> rec3(3 * V);
+        for (long v = 0; v < V; ++v)
+            for (int k = 0; k < 3; ++k) rec3[3 * v + k] = buf[k * V + v];
+        snprintf(name, sizeof(name), "%s_S%04d.iout", fin ? "new_position_final" : "new_position", step);
+        if ((rc = write_raw(dir, name, rec3.data(), rec3.size() * sizeof(double)))) return rc;
+    }
+    // work_e is dumped only at the end (evolution.c:603; its per-step call is commented out)
+    if (fin) {
+        if ((rc = voce_get_field(c, VOCE_FIELD_WORK_E, buf.data(), buf.size() * sizeof(double)))) return rc;
+        snprintf(name, sizeof(name), "work_e_final_S%06d.iout", step);
         if ((rc = write_raw(dir, name, buf.data(), V * sizeof(double)))) return rc;
     }
     return 0;

@@ -97,6 +97,10 @@ struct RveDev {
     long V, nspec;               // voxels, complex elements of one spectrum component (padded)
     double *E6, *S6, *P6, *EP5, *ED5, *CRSS, *CRSSB, *R, *WG, *WF, *GAC, *GTOT;
     double *GAMDOT, *TAUX, *SVM, *EVM, *WORKP, *RHOR, *RHOC;   // diagnostics (may be null)
+    // diagnostics: running sums of the Green-operator updates (the reference's
+    // `fluctuation`, evolution.c:392) that new_position reads: B-basis
+    // components 2, 3, 4, 6 of the symmetric part, then skew (2,1) and (0,2)
+    double *FLU;
     uint32_t *ORI;
     uint8_t *PH;
     double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1

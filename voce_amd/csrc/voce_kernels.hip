@@ -460,12 +460,26 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
             const double val = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
             E[e] = creep_set ? cr + (sh + val) : ev + (sh + val);
         });
+        // diagnostics: the running sum of the Green updates (the reference's
+        // fluctuation, evolution.c:392) for the components new_position reads
+        const int fs = (c == 1) ? 0 : (c == 2) ? 1 : (c == 3) ? 2 : (c == 5) ? 3 : -1;
+        if (r.FLU && fs >= 0) {
+            double *F = r.FLU + (long)fs * r.V + (long)x * ny * nz;
+            for (int e = tid; e < ny * nz; e += NT) {
+                int z;
+                const int y = bynz.div(e, z);
+                F[e] += (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
+            }
+        }
     } else {
         double *W = r.WF + off;
+        double *F = (r.FLU && c < 2) ? r.FLU + (long)(4 + c) * r.V + (long)x * ny * nz : nullptr;
         for (int e = tid; e < ny * nz; e += NT) {
             int z;
             const int y = bynz.div(e, z);
-            W[e] = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
+            const double val = (double)sr[2 * ((y >> 1) * 2 * nzh + z) + (y & 1)];
+            W[e] = val;
+            if (F) F[e] += val;
         }
     }
 }
