@@ -685,7 +685,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     };
     const size_t o_E6 = take(6 * V * db), o_S6 = take(6 * V * db), o_EP5 = take(5 * V * db), o_ED5 = take(5 * V * db);
     const size_t o_CR = take(12 * V * db), o_CRB = two_any ? take(12 * V * db) : 0;
-    const size_t o_R = take(9 * V * db), o_WG = take(3 * V * db), o_WF = take(3 * V * db), o_GAC = take(V * db);
+    const size_t o_R = take(9 * V * db), o_KC = take(21 * voce_kcv_voxels(V) * db), o_WG = take(3 * V * db), o_WF = take(3 * V * db), o_GAC = take(V * db);
     const size_t o_GT = take(12 * V * db), o_ORI = take(V * sizeof(uint32_t)), o_PH = take(V);
     const size_t o_SL = take(otab.size() * db), o_PC = take(2 * sizeof(PhaseConst));
     const size_t o_TX = take(nx * cb), o_TY = take(ny * cb), o_TZ = take(nz * cb), o_S6O = take(sig6o.size() * db);
@@ -703,6 +703,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.CRSS = (double *)(arena + o_CR);
     d.CRSSB = two_any ? (double *)(arena + o_CRB) : d.CRSS;
     d.R = (double *)(arena + o_R);
+    d.KCV = (double *)(arena + o_KC);
     d.WG = (double *)(arena + o_WG);
     d.WF = (double *)(arena + o_WF);
     d.GAC = (double *)(arena + o_GAC);

@@ -101,6 +101,12 @@ struct RveDev {
     // `fluctuation`, evolution.c:392) that new_position reads: B-basis
     // components 2, 3, 4, 6 of the symmetric part, then skew (2,1) and (0,2)
     double *FLU;
+    // K_c = C066^-1 + S_local of the voxel's orientation-table row in the
+    // crystal frame of its TranMat (Q6^T K_c Q6, Q6 the B-basis rotation),
+    // packed lower triangle, rewritten whenever TranMat is.  Tiles of 64
+    // voxels, component-major inside a tile: component q of voxel v at
+    // ((v / 64) * 21 + q) * 64 + v % 64 (21 * voce_kcv_voxels(V) doubles)
+    double *KCV;
     uint32_t *ORI;
     uint8_t *PH;
     double *SLOC;                // per orientation, VOCE_OTAB doubles: S_local = C_gr^-1
@@ -138,6 +144,9 @@ struct LaunchSet {
     size_t pl_lds, x_lds;        // dynamic LDS: the largest over the set's RVEs
     int fft_f32, uniform_phase, two_crss;
 };
+
+// voxels the KCV array is allocated for (whole tiles of 64)
+static inline long voce_kcv_voxels(long V) { return (V + 63) & ~63L; }
 
 #ifdef __cplusplus
 extern "C" {

@@ -743,42 +743,52 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
 // Q[a][b] = B_a : (R B_b R^T), a, b < 5.  A crystal-frame Schmid vector p
 // maps to the sample frame as Q p, so update_schmid (kinematics.c:426-459)
 // is Q applied to Schm_xt, and the resolved shear is p . (Q^T sigma).
-__device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5])
+// Column b of the B-basis rotation Q of TranMat R: the B components of
+// R B_b R^T, so that the components of R T R^T are Q t.
+__device__ __forceinline__ void basis_rotation_col(const double R[9], int b, double q[5])
 {
     // columns of R
     const double c0[3] = {R[0], R[3], R[6]}, c1[3] = {R[1], R[4], R[7]}, c2[3] = {R[2], R[5], R[8]};
+    // M = R B_b R^T, symmetric: entries 00 11 22 12 02 01
+    double m[6];
+#define DY(u, v, i, j) ((u)[i] * (v)[j] + (v)[i] * (u)[j])
+    if (b == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int i = k < 3 ? k : (k == 3 ? 1 : 0), j = k < 3 ? k : (k == 5 ? 1 : 2);
+            m[k] = RSQ2 * (c1[i] * c1[j] - c0[i] * c0[j]);
+        }
+    } else if (b == 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int i = k < 3 ? k : (k == 3 ? 1 : 0), j = k < 3 ? k : (k == 5 ? 1 : 2);
+            m[k] = RSQ6 * (2. * c2[i] * c2[j] - c0[i] * c0[j] - c1[i] * c1[j]);
+        }
+    } else {
+        const double *u = (b == 2) ? c1 : c0;
+        const double *w = (b == 4) ? c1 : c2;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int i = k < 3 ? k : (k == 3 ? 1 : 0), j = k < 3 ? k : (k == 5 ? 1 : 2);
+            m[k] = RSQ2 * DY(u, w, i, j);
+        }
+    }
+#undef DY
+    q[0] = RSQ2 * (m[1] - m[0]);
+    q[1] = RSQ6 * (2. * m[2] - m[0] - m[1]);
+    q[2] = 2. * RSQ2 * m[3];
+    q[3] = 2. * RSQ2 * m[4];
+    q[4] = 2. * RSQ2 * m[5];
+}
+
+__device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5])
+{
 #pragma unroll
     for (int b = 0; b < 5; ++b) {
-        // M = R B_b R^T, symmetric: entries 00 11 22 12 02 01
-        double m[6];
-#define DY(u, v, i, j) ((u)[i] * (v)[j] + (v)[i] * (u)[j])
-        if (b == 0) {
+        double q[5];
+        basis_rotation_col(R, b, q);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
-                m[q] = RSQ2 * (c1[i] * c1[j] - c0[i] * c0[j]);
-            }
-        } else if (b == 1) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
-                m[q] = RSQ6 * (2. * c2[i] * c2[j] - c0[i] * c0[j] - c1[i] * c1[j]);
-            }
-        } else {
-            const double *u = (b == 2) ? c1 : c0;
-            const double *w = (b == 4) ? c1 : c2;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int i = q < 3 ? q : (q == 3 ? 1 : 0), j = q < 3 ? q : (q == 5 ? 1 : 2);
-                m[q] = RSQ2 * DY(u, w, i, j);
-            }
-        }
-#undef DY
-        Q[0][b] = RSQ2 * (m[1] - m[0]);
-        Q[1][b] = RSQ6 * (2. * m[2] - m[0] - m[1]);
-        Q[2][b] = 2. * RSQ2 * m[3];
-        Q[3][b] = 2. * RSQ2 * m[4];
-        Q[4][b] = 2. * RSQ2 * m[5];
+        for (int a = 0; a < 5; ++a) Q[a][b] = q[a];
     }
 }
 
@@ -1199,6 +1209,44 @@ __device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6
     return bad ? 35 : 0;
 }
 
+// K_c (row-major 6x6, symmetric) in the crystal frame of the TranMat whose
+// B-basis rotation is Q: Q6^T K_c Q6 with Q6 = diag(Q, 1) (the hydrostatic
+// component is rotation invariant), written as its packed lower triangle,
+// in tiles of 64 voxels (component q of voxel v at ((v / 64) 21 + q) 64 + v % 64).
+__device__ __forceinline__ double *kcv_voxel(const RveDev &r, long v) { return r.KCV + (v >> 6) * (21 * 64) + (v & 63); }
+
+__device__ __forceinline__ void store_kc_crystal(const RveDev &r, long v, const double *Kc, const double Q[5][5])
+{
+    double *dst = kcv_voxel(r, v);
+    double M[5][5];   // deviatoric block of K_c times Q
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) t += Kc[a * 6 + b] * Q[b][j];
+            M[a][j] = t;
+        }
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) t += Q[a][i] * M[a][j];
+            dst[lt6(i, j) * 64] = t;
+        }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) t += Kc[30 + a] * Q[a][j];
+        dst[lt6(5, j) * 64] = t;
+    }
+    dst[lt6(5, 5) * 64] = Kc[35];
+}
+
 template <int NT, bool UNI, bool TWO>
 __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
@@ -1239,14 +1287,25 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                     if (TWO) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / r.CRSSB[q * V + v];
                 }
             }
+            // The Newton loop runs in the crystal frame of TranMat R, which is
+            // fixed during an increment: with Q the B-basis rotation of R and
+            // Q6 = diag(Q, 1), sigma_x = Q6^T sigma, and every sample-frame
+            // quantity of the step maps the same way (g_x = Q6^T g,
+            // K_x = Q6^T K Q6 = K_c,x + dt D_x).  The Schmid vectors are then
+            // the constant crystal-frame ones (Schm_xt, init.c:217-224) and
+            // K_c,x is the per-voxel table the close-out keeps; the loop
+            // never touches R.
             double sig[6], signorm = 0.0;
             {
-                double de[6];
+                double Rm[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+                double s0[6], w[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    sig[i] = r.S6[i * V + v];
-                    de[i] = r.E6[i * V + v] - (i < 5 ? r.EP5[i * V + v] : 0.0);
-                    signorm += sig[i] * sig[i];
+                    s0[i] = r.S6[i * V + v];
+                    w[i] = (i < 5 ? r.EP5[i * V + v] : 0.0) - r.E6[i * V + v];
+                    signorm += s0[i] * s0[i];
                 }
                 // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
                 // part of the compliance-form residual below, kept in LDS (a
@@ -1254,15 +1313,30 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 const cptr<double> S0b = msc->S066;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    double t = -de[i];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) t -= S0b[i * 6 + j] * sig[j];
-                    lds_w[i][threadIdx.x] = t;
+                    for (int j = 0; j < 6; ++j) w[i] -= S0b[i * 6 + j] * s0[j];
                 }
+                // Q^T, one column of Q at a time
+#pragma unroll
+                for (int b = 0; b < 5; ++b) {
+                    double q[5];
+                    basis_rotation_col(Rm, b, q);
+                    double ts = 0.0, tw = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) {
+                        ts += q[a] * s0[a];
+                        tw += q[a] * w[a];
+                    }
+                    sig[b] = ts;
+                    lds_w[b][threadIdx.x] = tw;
+                }
+                sig[5] = s0[5];
+                lds_w[5][threadIdx.x] = w[5];
             }
             signorm = sqrt(signorm);
             const double nsr = pc->nsr, g0 = pc->gam0;
             const int intn = pc->int_n, ni = (int)nsr;
+            const gptr<double> kcv = as_global(kcv_voxel(r, v));
             double erral = 1E-6;   // 10 * erroral (evolution.c:58-61)
             double ed[5];
             int iterl = 0;
@@ -1271,51 +1345,29 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 RELOAD_BARRIER();
 #pragma unroll
                 for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
-                double Rm[9];
-                // TranMat is re-read (an L2 hit) each Newton step: kept out of
-                // LDS, the block fits 3 waves per SIMD
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
-                // StrainRate_eval (constitutive.c:4-72): edot and the
-                // symmetric 5x5 d(edot)/d(sigma), one slip system at a time.
-                // With n' = R n and b' = R b the sample-frame Schmid tensor is
-                // sym(n' (x) b') (update_schmid, kinematics.c:426-459).
+                // StrainRate_eval (constitutive.c:4-72) in the crystal frame:
+                // edot and the symmetric 5x5 d(edot)/d(sigma), one slip
+                // system at a time
                 double D[15];
 #pragma unroll
                 for (int a = 0; a < 5; ++a) ed[a] = 0.0;
 #pragma unroll
                 for (int q = 0; q < 15; ++q) D[q] = 0.0;
-                // slip vectors of system s + 1 are loaded while s is computed
-                double bnn[3] = {pc->bn[0][0], pc->bn[0][1], pc->bn[0][2]};
-                double bbn[3] = {pc->bb[0][0], pc->bb[0][1], pc->bb[0][2]};
+                // the Schmid vector of system s + 1 is loaded while s is computed
+                double Pn[5];
+#pragma unroll
+                for (int a = 0; a < 5; ++a) Pn[a] = pc->px[0][a];
                 VOCE_PRAGMA(unroll VOCE_NR_SYS_UNROLL)
                 for (int s = 0; s < nsys; ++s) {
-                    RELOAD_BARRIER();   // slip vectors are re-read from the scalar cache
+                    RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
                     const double icf = lds_ic[s][threadIdx.x];
                     const double icb = TWO ? lds_ic[TWO ? 12 + s : s][threadIdx.x] : icf;
-                    const double bn[3] = {bnn[0], bnn[1], bnn[2]}, bb[3] = {bbn[0], bbn[1], bbn[2]};
+                    double P[5];
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) P[a] = Pn[a];
                     if (s + 1 < nsys) {
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            bnn[i] = pc->bn[s + 1][i];
-                            bbn[i] = pc->bb[s + 1][i];
-                        }
-                    }
-                    double n[3], bv[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        n[i] = Rm[i * 3 + 0] * bn[0] + Rm[i * 3 + 1] * bn[1] + Rm[i * 3 + 2] * bn[2];
-                        bv[i] = Rm[i * 3 + 0] * bb[0] + Rm[i * 3 + 1] * bb[1] + Rm[i * 3 + 2] * bb[2];
-                    }
-                    // B-basis components of sym(n' (x) b'), the sample-frame Schmid vector
-                    double P[5];
-                    {
-                        const double d0 = n[0] * bv[0], d1 = n[1] * bv[1], d2 = n[2] * bv[2];
-                        P[0] = RSQ2 * (d1 - d0);
-                        P[1] = RSQ6 * (2. * d2 - d0 - d1);
-                        P[2] = RSQ2 * (n[1] * bv[2] + n[2] * bv[1]);
-                        P[3] = RSQ2 * (n[0] * bv[2] + n[2] * bv[0]);
-                        P[4] = RSQ2 * (n[0] * bv[1] + n[1] * bv[0]);
+                        for (int a = 0; a < 5; ++a) Pn[a] = pc->px[s + 1][a];
                     }
                     // resolved shear: the 5-component dot of constitutive.c:26-27
                     double rss0 = 0.0;
@@ -1361,14 +1413,11 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 // sigma afresh every step: a running update of K_c sigma would
                 // carry the rounding of the large early steps (where dt edot and
                 // dt D x nearly cancel) into the converged stress.
-                // K_c is stored symmetric: one triangle (21 loads, issued
-                // together) serves the residual and the factorisation
-                const gptr<double> Kc = tab + 36;
+                // One triangle of K_c,x (21 loads, issued together) serves the
+                // residual and the factorisation.
                 double K[21];
 #pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int j = 0; j <= i; ++j) K[lt6(i, j)] = Kc[i * 6 + j];
+                for (int q = 0; q < 21; ++q) K[q] = kcv[q * 64];
                 double g[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
@@ -1395,24 +1444,54 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 }
                 erral = sqrt(cn) / signorm;
             }
+            // back to the sample frame: stress, the stress of the last
+            // flow-rule evaluation and its strain rate
+            double sig_old[6];
+            RELOAD_BARRIER();
+            {
+                double Rm[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+                double sx[5], ox[5], ex[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    sx[i] = sig[i];
+                    ox[i] = lds_old[i][threadIdx.x];
+                    ex[i] = ed[i];
+                    sig[i] = sig_old[i] = ed[i] = 0.0;
+                }
+                sig_old[5] = lds_old[5][threadIdx.x];
+                // Q t, one column of Q at a time
+#pragma unroll
+                for (int b = 0; b < 5; ++b) {
+                    double q[5];
+                    basis_rotation_col(Rm, b, q);
+#pragma unroll
+                    for (int a = 0; a < 5; ++a) {
+                        sig[a] += q[a] * sx[b];
+                        sig_old[a] += q[a] * ox[b];
+                        ed[a] += q[a] * ex[b];
+                    }
+                }
+            }
             // error partials of the last NR iteration (evolution.c:103-114):
             // |sigma - lambda| and |eps_tot(sigma_old) - eps|
-            double cs = 0.0, ce = 0.0, sig_old[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) sig_old[i] = lds_old[i][threadIdx.x];
+            double cs = 0.0, ce = 0.0;
+            RELOAD_BARRIER();
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
                 const double lam = r.S6[i * V + v];
                 cs += (sig[i] - lam) * (sig[i] - lam);
+                r.S6[i * V + v] = sig[i];
+                r.P6[i * V + v] = sig_old[i];
+            }
+            RELOAD_BARRIER();
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
                 double t = (i < 5 ? r.EP5[i * V + v] + ed[i] * dt : 0.0) - r.E6[i * V + v];
 #pragma unroll
                 for (int j = 0; j < 6; ++j) t += tab[i * 6 + j] * sig_old[j];
                 ce += t * t;
-            }
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                r.S6[i * V + v] = sig[i];
-                r.P6[i * V + v] = sig_old[i];
             }
             acc[0] = sqrt(cs);
             acc[1] = sqrt(ce);
@@ -1702,6 +1781,9 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
                 orientation(Rn, Rot);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) r.R[i * V + v] = Rn[i];
+                double Qn[5][5];
+                basis_rotation(Rn, Qn);
+                store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Qn);
             }
             // harden (constitutive.c:250-310), from step 3, and the
             // rho/work diagnostics that follow it (evolution.c:509-529)
@@ -1845,6 +1927,12 @@ __global__ void k_init_state(const RveDev *__restrict__ rtab, const double *sig6
     const PhaseConst *pc = r.phase + (r.PH[v] - 1);
     const double *row = r.SLOC + (long)(gas ? 0 : ori) * VOCE_OTAB + 72;
     for (int i = 0; i < 9; ++i) r.R[i * V + v] = gas ? (double)(i % 4 == 0) : row[i];
+    if (!gas) {
+        double Rm[9], Q[5][5];
+        for (int i = 0; i < 9; ++i) Rm[i] = row[i];
+        basis_rotation(Rm, Q);
+        store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
+    }
     for (int i = 0; i < 6; ++i) {
         r.S6[i * V + v] = gas ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
         r.E6[i * V + v] = 0.0;
