@@ -798,10 +798,12 @@ __device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5]
 // the scalar cache or L1; holding them would cost occupancy.
 #define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
 
-// slip systems per trip of the flow-rule loop of the Newton kernel
+// waves per SIMD the Newton kernel is compiled for (its LDS allows 3; with a
+// separate backward CRSS it allows 2)
 #ifndef VOCE_NR_WAVES
-#define VOCE_NR_WAVES 2      // waves per SIMD the Newton kernel is compiled for
+#define VOCE_NR_WAVES 3
 #endif
+// slip systems per trip of the flow-rule loop of the Newton kernel
 #ifndef VOCE_NR_SYS_UNROLL
 #define VOCE_NR_SYS_UNROLL 1
 #endif
@@ -916,12 +918,6 @@ __device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[
 #pragma unroll
             for (int b = a; b < 5; ++b) { D[q] += Q[a][c] * m[b]; ++q; }
     }
-}
-
-__device__ __forceinline__ double Dget(const double D[15], int i, int j)
-{
-    const int a = i < j ? i : j, b = i < j ? j : i;
-    return D[a * 5 - (a * (a - 1)) / 2 + (b - a)];
 }
 
 
@@ -1138,11 +1134,15 @@ __device__ void finalize_iteration(MacroState *ms, const MacroState *in, const d
 // with K_c = C0^-1 + S_local (per orientation, from the orientation table)
 // and w = (eps_p - eps) - C0^-1 lambda, so the step J^-1 R = K^-1 g is one
 // LDL^T solve of a symmetric positive definite 6x6 (see ldlt_solve6).
-// StrainRate_eval's sums (constitutive.c:49-64) enter one slip system at a
-// time from the rotated slip vectors n' = R n, b' = R b: the resolved shear
-// n' . sigma b', the Schmid vector P_s = B(sym(n' (x) b')), edot += P_s
-// gamdot_s and D += tau_x_s P_s (x) P_s, so the rotated Schmid set is never
-// held whole.
+// The iterations run in the crystal frame of the voxel's TranMat R (fixed
+// during an increment): with Q the B-basis rotation of R, every quantity of
+// the step maps by Q6 = diag(Q, 1), the sample-frame Schmid vectors
+// Schm_gr = Q Schm_xt (update_schmid, kinematics.c:426-459) become the
+// constant Schm_xt, and K_c becomes the per-voxel Q6^T K_c Q6 that the
+// close-out rewrites with R.  StrainRate_eval's sums (constitutive.c:49-64)
+// enter one slip system at a time: edot += P_s gamdot_s and
+// D += tau_x_s P_s (x) P_s.  Stress, the last flow-rule stress and edot are
+// rotated back once, after the loop.
 //
 // Partials per block: [0] sum|sigma-lambda|, [1] sum|eps_tot-eps|,
 // [2..7] sum sigma (B basis), [8..13] sum sigma over phase 1, summed in
@@ -1247,8 +1247,134 @@ __device__ __forceinline__ void store_kc_crystal(const RveDev &r, long v, const 
     dst[lt6(5, 5) * 64] = Kc[35];
 }
 
+// Forms of the power law x^(n-1), x^n of the flow rule: n = 20 (a fixed
+// product chain), another integer n (binary powering), real n (pow).
+enum { PM_N20 = 0, PM_INT = 1, PM_REAL = 2 };
+
+// Newton iterations of UpdateStress (evolution.c:63-114) on one voxel, in the
+// crystal frame: sig (crystal-frame stress, updated), ed (crystal-frame
+// strain rate of the last flow-rule evaluation), lds_old (the stress it was
+// evaluated at), lds_w (crystal-frame w), lds_ic (1 / CRSS).  Returns 0, or 35 (the reference's singular-matrix code) for a
+// non-positive pivot.
+template <int NT, bool TWO, int PM>
+__device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double signorm, cptr<PhaseConst> pc, int nsys,
+                                              double dt, gptr<double> kcv, double (*lds_w)[NT], double (*lds_old)[NT],
+                                              double (*lds_ic)[NT])
+{
+    const double nsr = pc->nsr, g0 = pc->gam0;
+    const int ni = (int)nsr;
+    double erral = 1E-6;   // 10 * erroral (evolution.c:58-61)
+    int iterl = 0;
+    while ((iterl < 100) && (fabs(erral) > 1E-7)) {
+        ++iterl;
+        RELOAD_BARRIER();
+#pragma unroll
+        for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
+        // StrainRate_eval (constitutive.c:4-72) in the crystal frame: edot
+        // and the symmetric 5x5 d(edot)/d(sigma), one slip system at a time
+        double D[15];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) ed[a] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 15; ++q) D[q] = 0.0;
+        // the Schmid vector and 1/CRSS of system s + 1 are loaded while s is computed
+        double Pn[5], icfn = lds_ic[0][threadIdx.x], icbn = TWO ? lds_ic[12][threadIdx.x] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) Pn[a] = pc->px[0][a];
+        VOCE_PRAGMA(unroll VOCE_NR_SYS_UNROLL)
+        for (int s = 0; s < nsys; ++s) {
+            RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
+            const double icf = icfn;
+            const double icb = TWO ? icbn : icf;
+            double P[5];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) P[a] = Pn[a];
+            if (s + 1 < nsys) {
+#pragma unroll
+                for (int a = 0; a < 5; ++a) Pn[a] = pc->px[s + 1][a];
+                icfn = lds_ic[s + 1][threadIdx.x];
+                if (TWO) icbn = lds_ic[TWO ? 13 + s : 0][threadIdx.x];
+            }
+            // resolved shear: the 5-component dot of constitutive.c:26-27
+            double rss0 = 0.0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) rss0 += P[a] * sig[a];
+            const double itc = (rss0 < 0) ? icb : icf;
+            const double rr = rss0 * itc;
+            double pn1, pn;
+            if (PM == PM_N20) {
+                // x^19 with ipow's products in ipow's order
+                const double ar = fabs(rr);
+                const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+                pn1 = (ar * x2) * x16;
+                pn = pn1 * ar;
+            } else if (PM == PM_INT) {
+                const double ar = fabs(rr);
+                pn1 = ipow(ar, ni - 1);
+                pn = pn1 * ar;
+            } else {
+                pn1 = fabs(pow(rr, nsr - 1.));
+                pn = fabs(pow(rr, nsr));
+            }
+            const double rss1 = g0 * nsr * pn1 * itc;
+            const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+#pragma unroll
+            for (int a = 0; a < 5; ++a) ed[a] += P[a] * rss2;
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const double t = P[a] * rss1;
+#pragma unroll
+                for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
+            }
+        }
+        RELOAD_BARRIER();
+        // Newton step x = J^-1 R (evolution.c:70-101) with
+        // J = I + C0 (S_local + dt D) and R = sigma - lambda + C0 (eps_tot - eps),
+        // eps_tot = eps_p_t + dt edot + S_local sigma.  Both carry the left
+        // factor C0 = C066: J = C0 K with K = K_c + dt D, K_c = S0 + S_local,
+        // and C0^-1 R = g = K_c sigma + w + dt edot.  K is symmetric positive
+        // definite (two compliances plus the PSD flow-rule Jacobian), so
+        // x = K^-1 g is an LDL^T solve without pivoting.  g is formed from
+        // sigma afresh every step: a running update of K_c sigma would
+        // carry the rounding of the large early steps (where dt edot and
+        // dt D x nearly cancel) into the converged stress.
+        // One triangle of K_c,x (21 loads, issued together) serves the
+        // residual and the factorisation.
+        double K[21];
+#pragma unroll
+        for (int q = 0; q < 21; ++q) K[q] = kcv[q * 64];
+        double g[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double t = lds_w[i][threadIdx.x] + (i < 5 ? dt * ed[i] : 0.0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
+            g[i] = t;
+        }
+        {
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a)
+#pragma unroll
+                for (int b2 = a; b2 < 5; ++b2) { K[lt6(b2, a)] += dt * D[q]; ++q; }
+        }
+        double x[6];
+        if (ldlt_solve6_packed(K, g, x)) return 35;
+        double cn = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double so = sig[i];
+            sig[i] -= x[i];
+            cn += (sig[i] - so) * (sig[i] - so);
+        }
+        erral = sqrt(cn) / signorm;
+    }
+    return 0;
+}
+
 template <int NT, bool UNI, bool TWO>
-__global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
+__global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     RVE_PROLOGUE(lb);
     MacroState *ms = r.ms;
@@ -1334,116 +1460,19 @@ __global__ __launch_bounds__(NT, TWO ? 1 : VOCE_NR_WAVES) void k_nr(const RveDev
                 lds_w[5][threadIdx.x] = w[5];
             }
             signorm = sqrt(signorm);
-            const double nsr = pc->nsr, g0 = pc->gam0;
-            const int intn = pc->int_n, ni = (int)nsr;
             const gptr<double> kcv = as_global(kcv_voxel(r, v));
-            double erral = 1E-6;   // 10 * erroral (evolution.c:58-61)
             double ed[5];
-            int iterl = 0;
-            while ((iterl < 100) && (fabs(erral) > 1E-7)) {
-                ++iterl;
-                RELOAD_BARRIER();
-#pragma unroll
-                for (int i = 0; i < 6; ++i) lds_old[i][threadIdx.x] = sig[i];
-                // StrainRate_eval (constitutive.c:4-72) in the crystal frame:
-                // edot and the symmetric 5x5 d(edot)/d(sigma), one slip
-                // system at a time
-                double D[15];
-#pragma unroll
-                for (int a = 0; a < 5; ++a) ed[a] = 0.0;
-#pragma unroll
-                for (int q = 0; q < 15; ++q) D[q] = 0.0;
-                // the Schmid vector of system s + 1 is loaded while s is computed
-                double Pn[5];
-#pragma unroll
-                for (int a = 0; a < 5; ++a) Pn[a] = pc->px[0][a];
-                VOCE_PRAGMA(unroll VOCE_NR_SYS_UNROLL)
-                for (int s = 0; s < nsys; ++s) {
-                    RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
-                    const double icf = lds_ic[s][threadIdx.x];
-                    const double icb = TWO ? lds_ic[TWO ? 12 + s : s][threadIdx.x] : icf;
-                    double P[5];
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) P[a] = Pn[a];
-                    if (s + 1 < nsys) {
-#pragma unroll
-                        for (int a = 0; a < 5; ++a) Pn[a] = pc->px[s + 1][a];
-                    }
-                    // resolved shear: the 5-component dot of constitutive.c:26-27
-                    double rss0 = 0.0;
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) rss0 += P[a] * sig[a];
-                    const double itc = (rss0 < 0) ? icb : icf;
-                    const double rr = rss0 * itc;
-                    double pn1, pn;
-                    if (intn) {
-                        const double ar = fabs(rr);
-                        if (ni == 20) {
-                            // x^19 with ipow's products in ipow's order
-                            const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
-                            pn1 = (ar * x2) * x16;
-                        } else {
-                            pn1 = ipow(ar, ni - 1);
-                        }
-                        pn = pn1 * ar;
-                    } else {
-                        pn1 = fabs(pow(rr, nsr - 1.));
-                        pn = fabs(pow(rr, nsr));
-                    }
-                    const double rss1 = g0 * nsr * pn1 * itc;
-                    const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) ed[a] += P[a] * rss2;
-                    int q = 0;
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) {
-                        const double t = P[a] * rss1;
-#pragma unroll
-                        for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
-                    }
-                }
-                RELOAD_BARRIER();
-                // Newton step x = J^-1 R (evolution.c:70-101) with
-                // J = I + C0 (S_local + dt D) and R = sigma - lambda + C0 (eps_tot - eps),
-                // eps_tot = eps_p_t + dt edot + S_local sigma.  Both carry the left
-                // factor C0 = C066: J = C0 K with K = K_c + dt D, K_c = S0 + S_local,
-                // and C0^-1 R = g = K_c sigma + w + dt edot.  K is symmetric positive
-                // definite (two compliances plus the PSD flow-rule Jacobian), so
-                // x = K^-1 g is an LDL^T solve without pivoting.  g is formed from
-                // sigma afresh every step: a running update of K_c sigma would
-                // carry the rounding of the large early steps (where dt edot and
-                // dt D x nearly cancel) into the converged stress.
-                // One triangle of K_c,x (21 loads, issued together) serves the
-                // residual and the factorisation.
-                double K[21];
-#pragma unroll
-                for (int q = 0; q < 21; ++q) K[q] = kcv[q * 64];
-                double g[6];
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double t = lds_w[i][threadIdx.x] + (i < 5 ? dt * ed[i] : 0.0);
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
-                    g[i] = t;
-                }
-#pragma unroll
-                for (int i = 0; i < 5; ++i)
-#pragma unroll
-                    for (int j = 0; j <= i; ++j) K[lt6(i, j)] += dt * Dget(D, i, j);
-                double x[6];
-                if (ldlt_solve6_packed(K, g, x)) {
-                    ms->error = 35;
-                    break;
-                }
-                double cn = 0.0;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    const double so = lds_old[i][threadIdx.x];
-                    sig[i] -= x[i];
-                    cn += (sig[i] - so) * (sig[i] - so);
-                }
-                erral = sqrt(cn) / signorm;
-            }
+            // the power law's form is uniform per phase: one loop per form
+            const int ni = (int)pc->nsr;
+            const int pm = pc->int_n ? (ni == 20 ? PM_N20 : PM_INT) : PM_REAL;
+            int err;
+            if (pm == PM_N20)
+                err = newton_crystal<NT, TWO, PM_N20>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+            else if (pm == PM_INT)
+                err = newton_crystal<NT, TWO, PM_INT>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+            else
+                err = newton_crystal<NT, TWO, PM_REAL>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+            if (err) ms->error = err;
             // back to the sample frame: stress, the stress of the last
             // flow-rule evaluation and its strain rate
             double sig_old[6];
