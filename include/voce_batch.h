@@ -111,7 +111,9 @@ typedef struct voce_result {
 int voce_server_create(const voce_server_opts *opts, voce_server **out);
 /* Register id (0 <= id < max_rves) with its RVE: builds its resident state. */
 int voce_server_add_rve(voce_server *s, int id, const voce_setup *setup);
-/* Queue one increment of id under velocity gradient L (row-major 3x3, 1/s). */
+/* Queue one increment of id under velocity gradient L (row-major 3x3, 1/s).
+ * VOCE_ERR_NOMEM: the id's per-increment workspace is larger than the
+ * whole workspace pool, so it could never be admitted. */
 int voce_server_submit(voce_server *s, int id, const double L[9], int audit, long tag);
 /* One scheduling round: admit, iterate one chunk, retire finished
  * increments.  *inflight (may be NULL) receives the number of increments

@@ -124,8 +124,8 @@ def test_admission_under_memory_pressure(tmp_path):
     for rid, L, au, tag in reqs:
         free.submit(rid, L, au, tag)
     rf = _by_tag(free.drain())
-    # room for about one 16^3 workspace at a time, at most two in flight
-    tight = _server(cases, max_inflight=2, workspace_bytes=400_000)
+    # room for about one 16^3 workspace (about 1 MB) at a time, at most two in flight
+    tight = _server(cases, max_inflight=2, workspace_bytes=1_300_000)
     for rid, L, au, tag in reqs:
         tight.submit(rid, L, au, tag)
     rt = _by_tag(tight.drain())
@@ -134,3 +134,11 @@ def test_admission_under_memory_pressure(tmp_path):
         _same(rf[t], rt[t])
     free.close()
     tight.close()
+    # a pool smaller than one 16^3 workspace: refused at submission, not queued for ever
+    tiny = _server(cases, workspace_bytes=300_000)
+    with pytest.raises(voce_amd.VoceError) as e:
+        tiny.submit(1, reqs[0][1], False, 0)
+    assert e.value.code == -5
+    tiny.submit(0, reqs[0][1], False, 1)   # an 8^3 workspace fits
+    assert [r["tag"] for r in tiny.drain()] == [1]
+    tiny.close()

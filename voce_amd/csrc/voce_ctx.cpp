@@ -685,13 +685,14 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     };
     const size_t o_E6 = take(6 * V * db), o_S6 = take(6 * V * db), o_EP5 = take(5 * V * db), o_ED5 = take(5 * V * db);
     const size_t o_CR = take(12 * V * db), o_CRB = two_any ? take(12 * V * db) : 0;
-    const size_t o_R = take(9 * V * db), o_KC = take(21 * voce_kcv_voxels(V) * db), o_WG = take(3 * V * db), o_WF = take(3 * V * db), o_GAC = take(V * db);
+    const size_t o_R = take(9 * V * db), o_WG = take(3 * V * db), o_WF = take(3 * V * db), o_GAC = take(V * db);
     const size_t o_GT = take(12 * V * db), o_ORI = take(V * sizeof(uint32_t)), o_PH = take(V);
     const size_t o_SL = take(otab.size() * db), o_PC = take(2 * sizeof(PhaseConst));
     const size_t o_TX = take(nx * cb), o_TY = take(ny * cb), o_TZ = take(nz * cb), o_S6O = take(sig6o.size() * db);
     const size_t o_TAB = take(sizeof(RveDev));
     const size_t o_MS = own_ms ? take(sizeof(MacroState)) : 0, o_LOG = own_ms ? take(3 * (size_t)c->log_cap * db) : 0;
-    const size_t o_P6 = ws ? take(6 * V * db) : 0, o_SP = ws ? take(6 * d.nspec * cb) : 0;
+    const size_t o_P6 = ws ? take(6 * V * db) : 0, o_KC = ws ? take(21 * voce_kcv_voxels(V) * db) : 0;
+    const size_t o_SP = ws ? take(6 * d.nspec * cb) : 0;
     const size_t o_US = ws ? take(3 * d.nspec * cb) : 0, o_PT = ws ? take((size_t)d.nblk_vox * 16 * db) : 0;
     char *arena = nullptr;
     ALLOC(arena, off);
@@ -703,7 +704,6 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.CRSS = (double *)(arena + o_CR);
     d.CRSSB = two_any ? (double *)(arena + o_CRB) : d.CRSS;
     d.R = (double *)(arena + o_R);
-    d.KCV = (double *)(arena + o_KC);
     d.WG = (double *)(arena + o_WG);
     d.WF = (double *)(arena + o_WF);
     d.GAC = (double *)(arena + o_GAC);
@@ -728,6 +728,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     ms0->log_cap = c->log_cap;
     if (ws) {
         d.P6 = (double *)(arena + o_P6);
+        d.KCV = (double *)(arena + o_KC);
         d.spec = arena + o_SP;
         d.usum = arena + o_US;
         d.part = (double *)(arena + o_PT);

@@ -101,9 +101,10 @@ struct RveDev {
     // `fluctuation`, evolution.c:392) that new_position reads: B-basis
     // components 2, 3, 4, 6 of the symmetric part, then skew (2,1) and (0,2)
     double *FLU;
-    // K_c = C066^-1 + S_local of the voxel's orientation-table row in the
-    // crystal frame of its TranMat (Q6^T K_c Q6, Q6 the B-basis rotation),
-    // packed lower triangle, rewritten whenever TranMat is.  Tiles of 64
+    // workspace: K_c = C066^-1 + S_local of the voxel's orientation-table
+    // row in the crystal frame of its TranMat (Q6^T K_c Q6, Q6 the B-basis
+    // rotation), packed lower triangle, written by the first Newton launch
+    // of each increment (TranMat is fixed until the close-out).  Tiles of 64
     // voxels, component-major inside a tile: component q of voxel v at
     // ((v / 64) * 21 + q) * 64 + v % 64 (21 * voce_kcv_voxels(V) doubles)
     double *KCV;

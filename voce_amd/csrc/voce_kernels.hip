@@ -1426,6 +1426,13 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                 double Rm[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+                if (msc->iter == 0) {
+                    // first iteration of the increment: K_c in the frame of
+                    // this increment's TranMat
+                    double Q[5][5];
+                    basis_rotation(Rm, Q);
+                    store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
+                }
                 double s0[6], w[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
@@ -1810,9 +1817,6 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
                 orientation(Rn, Rot);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) r.R[i * V + v] = Rn[i];
-                double Qn[5][5];
-                basis_rotation(Rn, Qn);
-                store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Qn);
             }
             // harden (constitutive.c:250-310), from step 3, and the
             // rho/work diagnostics that follow it (evolution.c:509-529)
@@ -1956,12 +1960,6 @@ __global__ void k_init_state(const RveDev *__restrict__ rtab, const double *sig6
     const PhaseConst *pc = r.phase + (r.PH[v] - 1);
     const double *row = r.SLOC + (long)(gas ? 0 : ori) * VOCE_OTAB + 72;
     for (int i = 0; i < 9; ++i) r.R[i * V + v] = gas ? (double)(i % 4 == 0) : row[i];
-    if (!gas) {
-        double Rm[9], Q[5][5];
-        for (int i = 0; i < 9; ++i) Rm[i] = row[i];
-        basis_rotation(Rm, Q);
-        store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
-    }
     for (int i = 0; i < 6; ++i) {
         r.S6[i * V + v] = gas ? 0.0 : sig6_by_orient[(long)ori * 6 + i];
         r.E6[i * V + v] = 0.0;

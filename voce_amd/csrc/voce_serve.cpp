@@ -111,7 +111,8 @@ struct Ws {
 size_t ws_bytes(const RveDev &d, bool f32)
 {
     const size_t cb = f32 ? 8 : 16, db = sizeof(double);
-    return al256(6 * d.V * db) + al256(6 * d.nspec * cb) + al256(3 * d.nspec * cb) + al256((size_t)d.nblk_vox * 16 * db);
+    return al256(6 * d.V * db) + al256(21 * voce_kcv_voxels(d.V) * db) + al256(6 * d.nspec * cb) +
+           al256(3 * d.nspec * cb) + al256((size_t)d.nblk_vox * 16 * db);
 }
 
 void ws_carve(RveDev &d, char *p)
@@ -119,6 +120,8 @@ void ws_carve(RveDev &d, char *p)
     const size_t cb = d.fft_f32 ? 8 : 16, db = sizeof(double);
     d.P6 = (double *)p;
     p += al256(6 * d.V * db);
+    d.KCV = (double *)p;
+    p += al256(21 * voce_kcv_voxels(d.V) * db);
     d.spec = p;
     p += al256(6 * d.nspec * cb);
     d.usum = p;
@@ -286,6 +289,7 @@ static void drop_ws(voce_server *s, int row, RveDev &d)
     if (w.held) s->pool.release(w.off, w.len);
     w.held = false;
     d.P6 = nullptr;
+    d.KCV = nullptr;
     d.spec = nullptr;
     d.usum = nullptr;
     d.part = nullptr;
@@ -363,6 +367,8 @@ extern "C" int voce_server_add_rve(voce_server *s, int id, const voce_setup *su)
 extern "C" int voce_server_submit(voce_server *s, int id, const double L[9], int audit, long tag)
 {
     if (!s || !L || id < 0 || id >= s->o.max_rves || !s->ctx[id]) return VOCE_ERR_ARG;
+    // a workspace larger than the whole pool would never be admitted
+    if (ws_bytes(s->ctx[id]->d, s->ctx[id]->d.fft_f32 != 0) > s->pool.cap) return VOCE_ERR_NOMEM;
     Request r;
     r.tag = tag;
     r.id = id;
@@ -389,10 +395,11 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     voce_spec_layout(&d);   // FP64 spectrum layout
     const size_t nspec = d.nspec;
     size_t need = 0;
-    const size_t sz_state[12] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
-                                 9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db, 21 * voce_kcv_voxels(V) * db};
+    const size_t sz_state[11] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
+                                 9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db};
     for (size_t b : sz_state) need += al256(b);
-    need += al256(6 * V * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) + al256((size_t)d0.nblk_vox * 16 * db);
+    need += al256(6 * V * db) + al256(21 * voce_kcv_voxels(V) * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) +
+            al256((size_t)d0.nblk_vox * 16 * db);
     need += al256(sizeof(MacroState)) + al256(3 * (size_t)c->log_cap * db);
     need += al256(d0.nx * cb) + al256(d0.ny * cb) + al256(d0.nz * cb);
     if (sh.arena && sh.bytes < need) {
@@ -410,19 +417,21 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     sh.V = V;
     d.GAMDOT = d.TAUX = d.SVM = d.EVM = d.WORKP = d.RHOR = d.RHOC = nullptr;
     char *p = sh.arena;
-    double *dst[12];
-    const double *srcp[12] = {d0.E6, d0.S6, d0.EP5, d0.ED5, d0.CRSS, two ? d0.CRSSB : nullptr, d0.R, d0.WG, d0.WF,
-                              d0.GAC, d0.GTOT, d0.KCV};
-    for (int i = 0; i < 12; ++i) {
+    double *dst[11];
+    const double *srcp[11] = {d0.E6, d0.S6, d0.EP5, d0.ED5, d0.CRSS, two ? d0.CRSSB : nullptr, d0.R, d0.WG, d0.WF,
+                              d0.GAC, d0.GTOT};
+    for (int i = 0; i < 11; ++i) {
         dst[i] = sz_state[i] ? (double *)p : nullptr;
         p += al256(sz_state[i]);
         if (sz_state[i]) SCK(hipMemcpyAsync(dst[i], srcp[i], sz_state[i], hipMemcpyDeviceToDevice, s->stream));
     }
     d.E6 = dst[0]; d.S6 = dst[1]; d.EP5 = dst[2]; d.ED5 = dst[3]; d.CRSS = dst[4];
     d.CRSSB = two ? dst[5] : dst[4];
-    d.R = dst[6]; d.WG = dst[7]; d.WF = dst[8]; d.GAC = dst[9]; d.GTOT = dst[10]; d.KCV = dst[11];
+    d.R = dst[6]; d.WG = dst[7]; d.WF = dst[8]; d.GAC = dst[9]; d.GTOT = dst[10];
     d.P6 = (double *)p;
     p += al256(6 * V * db);
+    d.KCV = (double *)p;
+    p += al256(21 * voce_kcv_voxels(V) * db);
     d.spec = p;
     p += al256(6 * nspec * cb);
     d.usum = p;
