@@ -691,7 +691,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     const size_t o_TX = take(nx * cb), o_TY = take(ny * cb), o_TZ = take(nz * cb), o_S6O = take(sig6o.size() * db);
     const size_t o_TAB = take(sizeof(RveDev));
     const size_t o_MS = own_ms ? take(sizeof(MacroState)) : 0, o_LOG = own_ms ? take(3 * (size_t)c->log_cap * db) : 0;
-    const size_t o_P6 = ws ? take(6 * V * db) : 0, o_KC = ws ? take(21 * voce_kcv_voxels(V) * db) : 0;
+    const size_t o_P6 = ws ? take(6 * V * db) : 0, o_KC = ws ? take(VOCE_KCV_COMP * voce_kcv_voxels(V) * db) : 0;
     const size_t o_SP = ws ? take(6 * d.nspec * cb) : 0;
     const size_t o_US = ws ? take(3 * d.nspec * cb) : 0, o_PT = ws ? take((size_t)d.nblk_vox * 16 * db) : 0;
     char *arena = nullptr;

@@ -106,7 +106,7 @@ struct RveDev {
     // rotation), packed lower triangle, written by the first Newton launch
     // of each increment (TranMat is fixed until the close-out).  Tiles of 64
     // voxels, component-major inside a tile: component q of voxel v at
-    // ((v / 64) * 21 + q) * 64 + v % 64 (21 * voce_kcv_voxels(V) doubles)
+    // ((v / 64) * 21 + q) * 64 + v % 64 (VOCE_KCV_COMP * voce_kcv_voxels(V) doubles)
     double *KCV;
     uint32_t *ORI;
     uint8_t *PH;
@@ -146,8 +146,11 @@ struct LaunchSet {
     int fft_f32, uniform_phase, two_crss;
 };
 
-// voxels the KCV array is allocated for (whole tiles of 64)
+// voxels the KCV array is allocated for (whole tiles of 64), and doubles per voxel
 static inline long voce_kcv_voxels(long V) { return (V + 63) & ~63L; }
+#ifndef VOCE_KCV_COMP
+#define VOCE_KCV_COMP 21
+#endif
 
 #ifdef __cplusplus
 extern "C" {

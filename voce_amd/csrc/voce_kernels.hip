@@ -1213,7 +1213,7 @@ __device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6
 // B-basis rotation is Q: Q6^T K_c Q6 with Q6 = diag(Q, 1) (the hydrostatic
 // component is rotation invariant), written as its packed lower triangle,
 // in tiles of 64 voxels (component q of voxel v at ((v / 64) 21 + q) 64 + v % 64).
-__device__ __forceinline__ double *kcv_voxel(const RveDev &r, long v) { return r.KCV + (v >> 6) * (21 * 64) + (v & 63); }
+__device__ __forceinline__ double *kcv_voxel(const RveDev &r, long v) { return r.KCV + (v >> 6) * (VOCE_KCV_COMP * 64) + (v & 63); }
 
 __device__ __forceinline__ void store_kc_crystal(const RveDev &r, long v, const double *Kc, const double Q[5][5])
 {

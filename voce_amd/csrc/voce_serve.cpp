@@ -111,7 +111,7 @@ struct Ws {
 size_t ws_bytes(const RveDev &d, bool f32)
 {
     const size_t cb = f32 ? 8 : 16, db = sizeof(double);
-    return al256(6 * d.V * db) + al256(21 * voce_kcv_voxels(d.V) * db) + al256(6 * d.nspec * cb) +
+    return al256(6 * d.V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db) + al256(6 * d.nspec * cb) +
            al256(3 * d.nspec * cb) + al256((size_t)d.nblk_vox * 16 * db);
 }
 
@@ -121,7 +121,7 @@ void ws_carve(RveDev &d, char *p)
     d.P6 = (double *)p;
     p += al256(6 * d.V * db);
     d.KCV = (double *)p;
-    p += al256(21 * voce_kcv_voxels(d.V) * db);
+    p += al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db);
     d.spec = p;
     p += al256(6 * d.nspec * cb);
     d.usum = p;
@@ -398,7 +398,7 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     const size_t sz_state[11] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
                                  9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db};
     for (size_t b : sz_state) need += al256(b);
-    need += al256(6 * V * db) + al256(21 * voce_kcv_voxels(V) * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) +
+    need += al256(6 * V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(V) * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) +
             al256((size_t)d0.nblk_vox * 16 * db);
     need += al256(sizeof(MacroState)) + al256(3 * (size_t)c->log_cap * db);
     need += al256(d0.nx * cb) + al256(d0.ny * cb) + al256(d0.nz * cb);
@@ -431,7 +431,7 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     d.P6 = (double *)p;
     p += al256(6 * V * db);
     d.KCV = (double *)p;
-    p += al256(21 * voce_kcv_voxels(V) * db);
+    p += al256(VOCE_KCV_COMP * voce_kcv_voxels(V) * db);
     d.spec = p;
     p += al256(6 * nspec * cb);
     d.usum = p;
