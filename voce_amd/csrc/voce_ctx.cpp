@@ -666,7 +666,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;
     // default FFT launch choices (a tuned set replaces them)
-    d.plv = (ny * d.nzh + ny / 2 <= 2144) ? 1 : 2;
+    d.plv = (ny * d.nzh <= 2112) ? 1 : 2;
     d.xv = 0;
     d.tk = voce_xgreen_tile(nx, 0);
     d.pl_r8 = d.x_r8 = 1;

@@ -200,7 +200,7 @@ struct FastDiv {
 // Element q of line l lives at s[l*lstride + q*estride].  All loads of a
 // stage complete (barrier) before any store, so the stage runs in place.
 // LF: consecutive lanes take consecutive lines: with an odd line stride
-// (x-pass and packed z rows) a stage's loads and stores hit distinct banks.
+// (the x pass) a stage's loads and stores hit distinct banks.
 // ------------------------------------------------------------------
 template <typename C, int R, int NT, int MAXEL, bool INV, bool LF>
 __device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int lstride, int estride,
@@ -325,11 +325,8 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
-    // packed z rows (two real rows as one complex row) at an odd stride:
-    // the consecutive rows that the lanes of a z stage take fall in
-    // different LDS banks
-    const int RS = 2 * nzh + 1;
-    C *twy = s + ny * nzh + ny / 2;
+    const int RS = 2 * nzh;   // packed z rows: two real rows as one complex row
+    C *twy = s + ny * nzh;
     C *twz = twy + ny;
     const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const C *gty = (const C *)r.tw_y;
@@ -345,7 +342,7 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         sr[2 * ((y >> 1) * RS + z) + (y & 1)] = (T)v;
     });
     __syncthreads();
-    fft_lines<C, NT, MAXEL, false, true>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
+    fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
     // split packed rows a + i b into the half spectra of a and b
     {
         constexpr int SPT = (MAXEL / 2 + NT - 1) / NT;
@@ -398,8 +395,8 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     extern __shared__ __align__(16) unsigned char smem[];
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
-    const int RS = 2 * nzh + 1;   // packed z rows at an odd stride, as in k_plane_fwd
-    C *twy = s + ny * nzh + ny / 2;
+    const int RS = 2 * nzh;   // packed z rows, as in k_plane_fwd
+    C *twy = s + ny * nzh;
     C *twz = twy + ny;
     const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const FastDiv bynz(nz);
@@ -451,7 +448,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         }
         __syncthreads();
     }
-    fft_lines<C, NT, MAXEL, true, true>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
+    fft_lines<C, NT, MAXEL, true, false>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
     const T *sr = (const T *)s;
     const long off = (long)c * r.V + (long)x * ny * nz;
     if (mode == 0) {
@@ -1996,14 +1993,12 @@ constexpr int NT_FIN = 256;
 // Launch-shape variants (threads per block, largest tile in complex
 // elements).  Plane kernels: ny * (nz/2+1) <= MAXEL.  x-pass kernel:
 // 6 * tk * nx <= MAXEL.  Which one runs for a grid size is tuned.
-// MAXEL: complex elements of plane data in LDS, ny (nz/2 + 1) + ny/2 (the
-// half spectra, or the packed rows at their odd stride)
 constexpr int NT_PL_T = 64, MAXEL_PL_T = 576;     // planes up to 32 x 32
-constexpr int NT_PL_S = 256, MAXEL_PL_S = 2144;   // planes up to 64 x 64
-constexpr int NT_PL_L = 512, MAXEL_PL_L = 8384;   // planes up to 128 x 128
+constexpr int NT_PL_S = 256, MAXEL_PL_S = 2112;   // planes up to 64 x 64
+constexpr int NT_PL_L = 512, MAXEL_PL_L = 8320;   // planes up to 128 x 128
 // 64 x 64 with five waves: the 33 half-spectrum columns of a 64 x 64 plane
 // are 264 radix-8 butterflies per stage, one per thread
-constexpr int NT_PL_M = 320, MAXEL_PL_M = 2144;
+constexpr int NT_PL_M = 320, MAXEL_PL_M = 2112;
 constexpr int NT_X = 256, MAXEL_X = 3072;
 constexpr int NT_XB = 128, MAXEL_XB = 1536;
 constexpr int NT_XC = 64, MAXEL_XC = 768;
@@ -2146,7 +2141,7 @@ static const size_t PLANE_STATIC_LDS = sizeof(MacroState) + 16 * 8 + (NT_PL_L / 
 
 extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 {
-    const int el = r->ny * r->nzh + r->ny / 2;
+    const int el = r->ny * r->nzh;
     if (plv < 0 || plv > 3 || el > PL_MAXEL[plv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
     const size_t b = (size_t)(el + r->ny + r->nz) * cb;

@@ -448,7 +448,7 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     p += al256(d0.ny * cb);
     d.tw_z = p;
     // FP64 launch choices (defaults; audits are not tuned)
-    d.plv = (d.ny * d.nzh + d.ny / 2 <= 2144) ? 1 : 2;
+    d.plv = (d.ny * d.nzh <= 2112) ? 1 : 2;
     d.xv = 0;
     d.tk = voce_xgreen_tile(d.nx, 0);
     d.pl_r8 = d.x_r8 = 1;
