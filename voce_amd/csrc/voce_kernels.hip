@@ -1485,21 +1485,23 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             if (err) ms->error = err;
             // back to the sample frame: stress, the stress of the last
             // flow-rule evaluation and its strain rate
-            double sig_old[6];
+            double sig_old[6], wsa[6];
             RELOAD_BARRIER();
             {
                 double Rm[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
-                double sx[5], ox[5], ex[5];
+                double sx[5], ox[5], ex[5], wx[5];
 #pragma unroll
                 for (int i = 0; i < 5; ++i) {
                     sx[i] = sig[i];
                     ox[i] = lds_old[i][threadIdx.x];
                     ex[i] = ed[i];
-                    sig[i] = sig_old[i] = ed[i] = 0.0;
+                    wx[i] = lds_w[i][threadIdx.x];
+                    sig[i] = sig_old[i] = ed[i] = wsa[i] = 0.0;
                 }
                 sig_old[5] = lds_old[5][threadIdx.x];
+                wsa[5] = lds_w[5][threadIdx.x];
                 // Q t, one column of Q at a time
 #pragma unroll
                 for (int b = 0; b < 5; ++b) {
@@ -1510,27 +1512,35 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                         sig[a] += q[a] * sx[b];
                         sig_old[a] += q[a] * ox[b];
                         ed[a] += q[a] * ex[b];
+                        wsa[a] += q[a] * wx[b];
                     }
                 }
             }
             // error partials of the last NR iteration (evolution.c:103-114):
-            // |sigma - lambda| and |eps_tot(sigma_old) - eps|
+            // |sigma - lambda| and |eps_tot(sigma_old) - eps|, the latter as
+            // w + C066^-1 lambda + dt edot + S_local sigma_old (w holds
+            // eps_p - eps - C066^-1 lambda)
             double cs = 0.0, ce = 0.0;
             RELOAD_BARRIER();
+            double lam[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                const double lam = r.S6[i * V + v];
-                cs += (sig[i] - lam) * (sig[i] - lam);
+                lam[i] = r.S6[i * V + v];
+                cs += (sig[i] - lam[i]) * (sig[i] - lam[i]);
                 r.S6[i * V + v] = sig[i];
                 r.P6[i * V + v] = sig_old[i];
             }
-            RELOAD_BARRIER();
+            {
+                const cptr<double> S0b = msc->S066;
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                double t = (i < 5 ? r.EP5[i * V + v] + ed[i] * dt : 0.0) - r.E6[i * V + v];
+                for (int i = 0; i < 6; ++i) {
+                    double t = wsa[i] + (i < 5 ? ed[i] * dt : 0.0);
 #pragma unroll
-                for (int j = 0; j < 6; ++j) t += tab[i * 6 + j] * sig_old[j];
-                ce += t * t;
+                    for (int j = 0; j < 6; ++j) t += S0b[i * 6 + j] * lam[j];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) t += tab[i * 6 + j] * sig_old[j];
+                    ce += t * t;
+                }
             }
             acc[0] = sqrt(cs);
             acc[1] = sqrt(ce);
