@@ -433,7 +433,9 @@ extern "C" int voce_batch_increment(voce_batch *b, voce_incr_report *reps)
     all.map_one = b->map_all;
     if (voce_launch_begin_increment(&all, b->stream)) return VOCE_ERR_DEVICE;
     int launched = 0;
-    int chunk = b->pred;
+    // the previous increment's largest iteration count, then chunks of 4, 8
+    // and 16 (one captured graph per length)
+    int chunk = b->pred, next_chunk = 4;
     while (launched < b->iter_max_any) {
         hipGraphExec_t ex;
         if ((rc = graph_for(b, chunk, &ex))) return rc;
@@ -447,7 +449,8 @@ extern "C" int voce_batch_increment(voce_batch *b, voce_incr_report *reps)
         if (hipStreamSynchronize(b->stream) != hipSuccess) return VOCE_ERR_DEVICE;
         if (b->h_cnt[1]) return b->h_cnt[1];
         if (b->h_cnt[0] == 0) break;
-        chunk = 4;
+        chunk = next_chunk;
+        next_chunk = next_chunk < 16 ? 2 * next_chunk : 16;
     }
     for (Group &g : b->groups)
         if (voce_launch_closeout(&g.L, b->stream, 1)) return VOCE_ERR_DEVICE;

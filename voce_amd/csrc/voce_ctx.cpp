@@ -830,7 +830,11 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
         CK(hipMemcpyAsync(c->h_ctl, (char *)d.ms + ctl_off, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         CK(hipStreamSynchronize(c->stream));
     }
+    // first chunk: the previous increment's iteration count; past it (or
+    // with no history) chunks of 2, 4, 8 and then 16, so a long increment
+    // costs few host round trips and at most 15 launches past convergence
     int chunk = c->pred_iters > 0 ? c->pred_iters : 1;
+    int next_chunk = 2;
     int done_before = 0;
     while (launched < imax && c->h_ctl[3]) {
         int n = chunk;
@@ -870,7 +874,8 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
             done_before = c->h_ctl[1];
         }
         if (!c->h_ctl[3]) break;
-        chunk = 2;
+        chunk = next_chunk;
+        next_chunk = next_chunk < 16 ? 2 * next_chunk : 16;
     }
     const int istep = c->h_ctl[0];
     const int need_skew = (c->update_flag == 1 && istep > 1) || c->creep || c->diag;
