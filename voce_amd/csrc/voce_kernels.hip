@@ -1280,24 +1280,9 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
         for (int a = 0; a < 5; ++a) ed[a] = 0.0;
 #pragma unroll
         for (int q = 0; q < 15; ++q) D[q] = 0.0;
-        // the Schmid vector and 1/CRSS of system s + 1 are loaded while s is computed
-        double Pn[5], icfn = lds_ic[0][threadIdx.x], icbn = TWO ? lds_ic[12][threadIdx.x] : 0.0;
-#pragma unroll
-        for (int a = 0; a < 5; ++a) Pn[a] = pc->px[0][a];
-        VOCE_PRAGMA(unroll VOCE_NR_SYS_UNROLL)
-        for (int s = 0; s < nsys; ++s) {
-            RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
-            const double icf = icfn;
-            const double icb = TWO ? icbn : icf;
-            double P[5];
-#pragma unroll
-            for (int a = 0; a < 5; ++a) P[a] = Pn[a];
-            if (s + 1 < nsys) {
-#pragma unroll
-                for (int a = 0; a < 5; ++a) Pn[a] = pc->px[s + 1][a];
-                icfn = lds_ic[s + 1][threadIdx.x];
-                if (TWO) icbn = lds_ic[TWO ? 13 + s : 0][threadIdx.x];
-            }
+        // two slip systems per trip: their independent dependency chains
+        // (resolved shear, power law) interleave
+        auto rates = [&](const double *P, double icf, double icb, double &rss1, double &rss2) {
             // resolved shear: the 5-component dot of constitutive.c:26-27
             double rss0 = 0.0;
 #pragma unroll
@@ -1319,8 +1304,10 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
                 pn1 = fabs(pow(rr, nsr - 1.));
                 pn = fabs(pow(rr, nsr));
             }
-            const double rss1 = g0 * nsr * pn1 * itc;
-            const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+            rss1 = g0 * nsr * pn1 * itc;
+            rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+        };
+        auto accum = [&](const double *P, double rss1, double rss2) {
 #pragma unroll
             for (int a = 0; a < 5; ++a) ed[a] += P[a] * rss2;
             int q = 0;
@@ -1330,6 +1317,35 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
 #pragma unroll
                 for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
             }
+        };
+        int s = 0;
+        for (; s + 1 < nsys; s += 2) {
+            RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
+            double P0[5], P1[5];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                P0[a] = pc->px[s][a];
+                P1[a] = pc->px[s + 1][a];
+            }
+            const double f0 = lds_ic[s][threadIdx.x], f1 = lds_ic[s + 1][threadIdx.x];
+            const double b0 = TWO ? lds_ic[TWO ? 12 + s : 0][threadIdx.x] : f0;
+            const double b1 = TWO ? lds_ic[TWO ? 13 + s : 0][threadIdx.x] : f1;
+            double r10, r20, r11, r21;
+            rates(P0, f0, b0, r10, r20);
+            rates(P1, f1, b1, r11, r21);
+            accum(P0, r10, r20);
+            accum(P1, r11, r21);
+        }
+        if (s < nsys) {
+            RELOAD_BARRIER();
+            double P0[5];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) P0[a] = pc->px[s][a];
+            const double f0 = lds_ic[s][threadIdx.x];
+            const double b0 = TWO ? lds_ic[TWO ? 12 + s : 0][threadIdx.x] : f0;
+            double r10, r20;
+            rates(P0, f0, b0, r10, r20);
+            accum(P0, r10, r20);
         }
         RELOAD_BARRIER();
         // Newton step x = J^-1 R (evolution.c:70-101) with
