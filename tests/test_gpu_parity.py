@@ -114,6 +114,14 @@ def test_fp64_parity_two_crystal_phases_generic_exponent(tmp_path):
     compare(tmp_path, c, 3)
 
 
+def test_fp64_parity_odd_slip_count(tmp_path):
+    # eleven slip systems: the Newton kernel's flow-rule loop takes systems
+    # in pairs, so an odd count runs its one-system tail
+    m = inputs.Material(slip=list(inputs.FCC_SLIP[:11]))
+    c = inputs.Case(dims=(8, 8, 8), n_grains=5, seed=41, n_steps=3, material=m)
+    compare(tmp_path, c, 3)
+
+
 def test_fp64_parity_creep(tmp_path):
     # every component stress-controlled (CREEP_FLAG, init.c:736-740)
     c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=35, n_steps=3, time_step=0.1,
