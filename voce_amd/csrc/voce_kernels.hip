@@ -12,6 +12,7 @@
 //                E += dDisGradAvg + IFFT/N (evolution.c:383-394).
 //   k_nr         one voxel per lane: the 6x6 Newton-Raphson of UpdateStress
 //                (evolution.c:3-168) with StrainRate_eval (constitutive.c:4-72),
+//                iterated in the crystal frame of the voxel's TranMat and
 //                solved in compliance form by an LDL^T factorisation in
 //                registers (in place of LU_inv_66, kinematics.c:282-424);
 //                wavefront-shuffle partial sums of the error norms and
