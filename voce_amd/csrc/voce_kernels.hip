@@ -925,7 +925,9 @@ __device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[
 }
 
 
-template <int NT, int NV>
+// DUP: out[NV .. NV + 5] repeat out[2 .. 7] (with one crystalline phase the
+// phase-1 stress sums are the stress sums)
+template <int NT, int NV, bool DUP = false>
 __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *out)
 {
     __shared__ double red[NT / 64][NV];
@@ -946,6 +948,7 @@ __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *o
 #pragma unroll
         for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
         out[threadIdx.x] = acc;
+        if (DUP && threadIdx.x >= 2) out[threadIdx.x + 6] = acc;
     }
 }
 
@@ -1568,7 +1571,15 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             }
         }
     }
-    block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
+    if (UNI) {
+        // every crystalline voxel is phase 1 (gas voxels add zeros to both)
+        double a8[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a8[i] = acc[i];
+        block_reduce_store<NT, 8, true>(a8, r.part + (long)lb * 16);
+    } else {
+        block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
+    }
     if (lb == 0 && threadIdx.x == 0) ms->fin_pending = 1;   // read after this kernel ends
 }
 
