@@ -15,7 +15,7 @@
 //                iterated in the crystal frame of the voxel's TranMat and
 //                solved in compliance form by an LDL^T factorisation in
 //                registers (in place of LU_inv_66, kinematics.c:282-424);
-//                wavefront-shuffle partial sums of the error norms and
+//                DPP wave sums (partial sums) of the error norms and
 //                stress averages per block.  One extra block of the next
 //                forward-plane launch (or k_fin_iter after the last launched
 //                iteration) sums them in fixed order and runs get_smacro
@@ -925,6 +925,30 @@ __device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[
 }
 
 
+// x moved across lanes by a DPP control (both 32-bit halves); lanes whose
+// row is masked off receive 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xf, false),
+                            __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, false));
+}
+
+// Sum of x over the 64 lanes of a wave, complete in lane 63: DPP moves in a
+// fixed order (quad swaps, half-row and row mirrors, row broadcasts), so
+// the result does not depend on timing and no LDS round trip is taken.
+__device__ __forceinline__ double wave_sum_last(double x)
+{
+    x += dpp_f64<0xB1, 0xf>(x);    // quad_perm [1,0,3,2]
+    x += dpp_f64<0x4E, 0xf>(x);    // quad_perm [2,3,0,1]
+    x += dpp_f64<0x141, 0xf>(x);   // row_half_mirror
+    x += dpp_f64<0x140, 0xf>(x);   // row_mirror: every lane holds its row's sum
+    x += dpp_f64<0x142, 0xa>(x);   // row_bcast:15 into rows 1 and 3
+    x += dpp_f64<0x143, 0xc>(x);   // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
 // DUP: out[NV .. NV + 5] repeat out[2 .. 7] (with one crystalline phase the
 // phase-1 stress sums are the stress sums)
 template <int NT, int NV, bool DUP = false>
@@ -932,14 +956,9 @@ __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *o
 {
     __shared__ double red[NT / 64][NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double x = vals[i];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        vals[i] = x;
-    }
+    for (int i = 0; i < NV; ++i) vals[i] = wave_sum_last(vals[i]);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
+    if (lane == 63)
 #pragma unroll
         for (int i = 0; i < NV; ++i) red[w][i] = vals[i];
     __syncthreads();
