@@ -1061,29 +1061,39 @@ orc_job *orc_setup(const char *input_path, const char *base_dir, int *err)
     return J;
 }
 
+/* update_schmid body for one voxel, kinematics.c:436-456: every system's
+ * crystal-frame Schmid vector (5 B-basis components) to a tensor, rotated
+ * with the voxel's TranMat (crystal -> sample) as R P R^T, and back. */
+void orc_update_schmid_voxel(const double tran[9], const double *schm_xt, int nsys, double *out)
+{
+    int i, j, mi, mj, mip, mjp;
+    double aux5[5];
+    t2 aux33, aux33r;
+    for (i = 0; i < nsys; i++) {
+        for (j = 0; j < 5; j++) aux5[j] = schm_xt[i * 5 + j];
+        chg_basis5(aux5, aux33, NULL, NULL, 1);
+        for (mi = 0; mi < 3; mi++)
+            for (mj = 0; mj < 3; mj++) {
+                aux33r[mi][mj] = 0.0;
+                for (mip = 0; mip < 3; mip++)
+                    for (mjp = 0; mjp < 3; mjp++)
+                        aux33r[mi][mj] += tran[mi * 3 + mip] * tran[mj * 3 + mjp] * aux33[mip][mjp];
+            }
+        chg_basis5(aux5, aux33r, NULL, NULL, 2);
+        for (j = 0; j < 5; j++) out[i * 5 + j] = aux5[j];
+    }
+}
+
 /* update_schmid, kinematics.c:426-459 */
 static void update_schmid(orc_job *J)
 {
     int p;
 #pragma omp parallel for schedule(static)
     for (p = 0; p < J->V; p++) {
-        int jph = J->phase_f[p], i, j, mi, mj, mip, mjp;
-        double aux5[5];
-        t2 aux33, aux33r;
+        int jph = J->phase_f[p];
         if (J->nm.Type_phases[jph - 1]) continue;
-        for (i = 0; i < J->nSYS[jph - 1]; i++) {
-            for (j = 0; j < 5; j++) aux5[j] = J->Schm_xt[jph - 1][i][j];
-            chg_basis5(aux5, aux33, NULL, NULL, 1);
-            for (mi = 0; mi < 3; mi++)
-                for (mj = 0; mj < 3; mj++) {
-                    aux33r[mi][mj] = 0.0;
-                    for (mip = 0; mip < 3; mip++)
-                        for (mjp = 0; mjp < 3; mjp++)
-                            aux33r[mi][mj] += J->TranMat[p][mi][mip] * J->TranMat[p][mj][mjp] * aux33[mip][mjp];
-                }
-            chg_basis5(aux5, aux33r, NULL, NULL, 2);
-            for (j = 0; j < 5; j++) J->Schm_gr[p][i][j] = aux5[j];
-        }
+        orc_update_schmid_voxel(&J->TranMat[p][0][0], &J->Schm_xt[jph - 1][0][0], J->nSYS[jph - 1],
+                                &J->Schm_gr[p][0][0]);
     }
 }
 
@@ -1233,13 +1243,53 @@ static int update_stress(orc_job *J, double *Err_e_local, double *Err_s_local)
     return 0;
 }
 
+/* Mixed-BC strain correction of get_smacro, evolution.c:199-211: for every
+ * Voigt component i that is not velocity-controlled, the entry (IJV[0][i],
+ * IJV[1][i]) -- upper triangle only -- of dD gets S0 contracted with the
+ * stress misfit of the stress-controlled Voigt components (again only
+ * their upper-triangle entries).  Every other entry of dD is left as is,
+ * as in the reference (it only writes these six). */
+void orc_bc_correction(const double S0[81], const int vflag[6], const int sflag[6], const double Scauchy[9],
+                       const double SigAvg[9], double dD[9])
+{
+    static const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+    int i, k, ii, jj, kk, ll;
+    for (i = 0; i < 6; i++) {
+        ii = IJV[0][i];
+        jj = IJV[1][i];
+        dD[ii * 3 + jj] = 0.0;
+        if (vflag[i] == 0) {
+            for (k = 0; k < 6; k++) {
+                kk = IJV[0][k];
+                ll = IJV[1][k];
+                dD[ii * 3 + jj] += S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * sflag[k] *
+                                   (Scauchy[kk * 3 + ll] - SigAvg[kk * 3 + ll]);
+            }
+        }
+    }
+}
+
+/* Error normalisation of Evolution, evolution.c:410-423: the voxel sums
+ * times WGT, Err_e divided by e_vm (the previous increment's macroscopic
+ * strain, d_vm*dt at the first, init.c:536) only when |e_vm| > 1e-5,
+ * Err_s divided by the s_vm get_smacro has just formed. */
+void orc_err_normalise(double err_e_sum, double err_s_sum, double wgt, double e_vm, double s_vm, double out[2])
+{
+    double Err_e = err_e_sum, Err_s = err_s_sum;
+    Err_e *= wgt;
+    Err_s *= wgt;
+    if (fabs(e_vm) > 1E-5) Err_e /= e_vm;
+    Err_s /= s_vm;
+    out[0] = Err_e;
+    out[1] = Err_s;
+}
+
 /* get_smacro, evolution.c:170-241 */
 static void get_smacro(orc_job *J)
 {
     double local_sigavg, local_sigavg1, sav6[6], sav5[5], v;
     t2 SigDevAvg;
-    int i, ii, jj, k, kk, ll, mi, mj, p;
-    static const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+    int i, mi, mj, p;
     for (mi = 0; mi < 3; mi++)
         for (mj = 0; mj < 3; mj++) {
             local_sigavg = 0.0;
@@ -1253,19 +1303,8 @@ static void get_smacro(orc_job *J)
             J->SigAvg[mi][mj] *= J->WGT;
             J->SigAvg1[mi][mj] *= J->WGT / J->Wgt_ph1;
         }
-    for (i = 0; i < 6; i++) {
-        ii = IJV[0][i];
-        jj = IJV[1][i];
-        J->dDisGradAvg[ii][jj] = 0.0;
-        if (J->nm.VelGrad_BC_Flag[i] == 0) {
-            for (k = 0; k < 6; k++) {
-                kk = IJV[0][k];
-                ll = IJV[1][k];
-                J->dDisGradAvg[ii][jj] += J->S0[ii][jj][kk][ll] * J->nm.Stress_BC_Flag[k] *
-                                          (J->Scauchy[kk][ll] - J->SigAvg[kk][ll]);
-            }
-        }
-    }
+    orc_bc_correction(&J->S0[0][0][0][0], J->nm.VelGrad_BC_Flag, J->nm.Stress_BC_Flag, &J->Scauchy[0][0],
+                      &J->SigAvg[0][0], &J->dDisGradAvg[0][0]);
     for (mi = 0; mi < 3; mi++)
         for (mj = 0; mj < 3; mj++) J->dDisGradAvg_acum[mi][mj] += J->dDisGradAvg[mi][mj];
     chg_basis(sav6, J->SigAvg, NULL, NULL, 2);
@@ -1298,41 +1337,76 @@ static double von_mises(t2 t)
     return (sqrt(2. / 3. * vm));
 }
 
+/* update_orient body for one voxel, constitutive.c:188-229: local spin
+ * skew(VelGrad), plastic spin skew(sum_s (R bb_s) (x) (R bn_s) gamdot_s)
+ * with R = TranMat (crystal -> sample), rotation (Udot_a + W_loc - W_p) dt
+ * applied by Orientation. */
+void orc_update_orient_voxel(double tran[9], const double velgrad[9], const double udot_a[9], const double *bn,
+                             const double *bb, const double *gamdot, int nsys, double dt)
+{
+    int is, i, j, mi, mj;
+    t2 tranmat, RotLoc, Lp, RotSlip, Rot;
+    double bb_sa[3], bn_sa[3];
+    for (mi = 0; mi < 3; mi++)
+        for (mj = 0; mj < 3; mj++) {
+            RotLoc[mi][mj] = (velgrad[mi * 3 + mj] - velgrad[mj * 3 + mi]) / 2.;
+            Lp[mi][mj] = 0.0;
+            tranmat[mi][mj] = tran[mi * 3 + mj];
+        }
+    for (is = 0; is < nsys; is++) {
+        for (i = 0; i < 3; i++) {
+            bb_sa[i] = 0.0;
+            bn_sa[i] = 0.0;
+            for (j = 0; j < 3; j++) {
+                bb_sa[i] += tranmat[i][j] * bb[is * 3 + j];
+                bn_sa[i] += tranmat[i][j] * bn[is * 3 + j];
+            }
+        }
+        for (mi = 0; mi < 3; mi++)
+            for (mj = 0; mj < 3; mj++) Lp[mi][mj] += bb_sa[mi] * bn_sa[mj] * gamdot[is];
+    }
+    for (mi = 0; mi < 3; mi++)
+        for (mj = 0; mj < 3; mj++) RotSlip[mi][mj] = (Lp[mi][mj] - Lp[mj][mi]) / 2.;
+    for (mi = 0; mi < 3; mi++)
+        for (mj = 0; mj < 3; mj++) Rot[mi][mj] = (udot_a[mi * 3 + mj] + RotLoc[mi][mj] - RotSlip[mi][mj]) * dt;
+    orientation(tranmat, Rot);
+    memcpy(tran, tranmat, sizeof(t2));
+}
+
 /* update_orient, constitutive.c:171-248 (diagnostic averages dropped) */
 static void update_orient(orc_job *J)
 {
     int p;
 #pragma omp parallel for schedule(static)
     for (p = 0; p < J->V; p++) {
-        int jph = J->phase_f[p], is, i, j, mi, mj;
-        t2 tranmat, RotLoc, Lp, RotSlip, Rot;
-        double bb_sa[3], bn_sa[3];
+        int jph = J->phase_f[p];
         if (J->nm.Type_phases[jph - 1]) continue;
-        for (mi = 0; mi < 3; mi++)
-            for (mj = 0; mj < 3; mj++) {
-                RotLoc[mi][mj] = (J->VelGrad[p][mi][mj] - J->VelGrad[p][mj][mi]) / 2.;
-                Lp[mi][mj] = 0.0;
-                tranmat[mi][mj] = J->TranMat[p][mi][mj];
-            }
-        for (is = 0; is < J->nSYS[jph - 1]; is++) {
-            for (i = 0; i < 3; i++) {
-                bb_sa[i] = 0.0;
-                bn_sa[i] = 0.0;
-                for (j = 0; j < 3; j++) {
-                    bb_sa[i] += tranmat[i][j] * J->bb[jph - 1][is][j];
-                    bn_sa[i] += tranmat[i][j] * J->bn[jph - 1][is][j];
-                }
-            }
-            for (mi = 0; mi < 3; mi++)
-                for (mj = 0; mj < 3; mj++) Lp[mi][mj] += bb_sa[mi] * bn_sa[mj] * J->gamdot[p][is];
-        }
-        for (mi = 0; mi < 3; mi++)
-            for (mj = 0; mj < 3; mj++) RotSlip[mi][mj] = (Lp[mi][mj] - Lp[mj][mi]) / 2.;
-        for (mi = 0; mi < 3; mi++)
-            for (mj = 0; mj < 3; mj++)
-                Rot[mi][mj] = (J->Udot_a[mi][mj] + RotLoc[mi][mj] - RotSlip[mi][mj]) * J->nm.TimeStep;
-        orientation(tranmat, Rot);
-        memcpy(J->TranMat[p], tranmat, sizeof(t2));
+        orc_update_orient_voxel(&J->TranMat[p][0][0], &J->VelGrad[p][0][0], &J->Udot_a[0][0],
+                                &J->bn[jph - 1][0][0], &J->bb[jph - 1][0][0], J->gamdot[p], J->nSYS[jph - 1],
+                                J->nm.TimeStep);
+    }
+}
+
+/* rho/work diagnostics of one voxel, evolution.c:509-529: with the
+ * resolved shear rss_s = Schm_gr_s . chg_basis(Sig) (5 B-basis terms),
+ * rho_rss = sum (rss_s/c)^2, rho_crss = sum (crss_s,fwd/c)^2 with
+ * c = 0.35*47.565*0.256, work_p = sum |gammatotal_s rss_s|.
+ * crss: nsys (forward, backward) pairs.  out: rho_rss, rho_crss, work_p. */
+void orc_rho_work_voxel(const double sig[9], const double *schm_gr, const double *crss, const double *gamtot,
+                        int nsys, double out[3])
+{
+    double str[6], rss;
+    t2 s33;
+    int i;
+    memcpy(s33, sig, sizeof(t2));
+    out[0] = out[1] = out[2] = 0.0;
+    chg_basis(str, s33, NULL, NULL, 2);
+    for (i = 0; i < nsys; i++) {
+        const double *sc = schm_gr + 5 * i;
+        rss = sc[0] * str[0] + sc[1] * str[1] + sc[2] * str[2] + sc[3] * str[3] + sc[4] * str[4];
+        out[0] += pow(rss / (0.35 * 47.565 * 0.256), 2);
+        out[1] += pow(crss[2 * i] / (0.35 * 47.565 * 0.256), 2);
+        out[2] += fabs((gamtot[i]) * rss);
     }
 }
 
@@ -1342,21 +1416,17 @@ static void harden_all(orc_job *J)
     int p;
 #pragma omp parallel for schedule(static)
     for (p = 0; p < J->V; p++) {
-        int jph = J->phase_f[p], i;
-        double str[6], rss;
+        int jph = J->phase_f[p];
         if (J->nm.Type_phases[jph - 1]) continue;
         harden_voxel(J->crss[p], &J->gamacum[p], J->gammatotal[p], J->gamdot[p], J->nSYS[jph - 1],
                      J->Hard[jph - 1], J->tau[jph - 1], J->thet[jph - 1], J->nm.TimeStep);
-        J->rho_rss[p] = 0.0;
-        J->rho_crss[p] = 0.0;
-        J->work_p[p] = 0.0;
-        chg_basis(str, J->Sig[p], NULL, NULL, 2);
-        for (i = 0; i < J->nSYS[jph - 1]; i++) {
-            const double *sc = J->Schm_gr[p][i];
-            rss = sc[0] * str[0] + sc[1] * str[1] + sc[2] * str[2] + sc[3] * str[3] + sc[4] * str[4];
-            J->rho_rss[p] += pow(rss / (0.35 * 47.565 * 0.256), 2);
-            J->rho_crss[p] += pow(J->crss[p][i][0] / (0.35 * 47.565 * 0.256), 2);
-            J->work_p[p] += fabs((J->gammatotal[p][i]) * rss);
+        {
+            double o3[3];
+            orc_rho_work_voxel(&J->Sig[p][0][0], &J->Schm_gr[p][0][0], &J->crss[p][0][0], J->gammatotal[p],
+                               J->nSYS[jph - 1], o3);
+            J->rho_rss[p] = o3[0];
+            J->rho_crss[p] = o3[1];
+            J->work_p[p] = o3[2];
         }
     }
 }
@@ -1433,13 +1503,13 @@ int orc_increment(orc_job *J, orc_report *rep)
         Err_s_local = 0.0;
         rc = update_stress(J, &Err_e_local, &Err_s_local);
         if (rc) return rc;
-        Err_e = Err_e_local;
-        Err_s = Err_s_local;
-        Err_e *= J->WGT;
-        Err_s *= J->WGT;
         get_smacro(J);
-        if (fabs(J->e_vm) > 1E-5) Err_e /= J->e_vm;
-        Err_s /= J->s_vm;
+        {
+            double en[2];
+            orc_err_normalise(Err_e_local, Err_s_local, J->WGT, J->e_vm, J->s_vm, en);
+            Err_e = en[0];
+            Err_s = en[1];
+        }
         if (rep && iter <= ORC_MAXIT) {
             rep->err_e[iter - 1] = Err_e;
             rep->err_s[iter - 1] = Err_s;
@@ -1592,6 +1662,10 @@ int orc_get_field(orc_job *J, int field, double *out)
         free(d);
         return 3;
     }
+    case 22: case 23: /* per-voxel error partials of the last UpdateStress, evolution.c:103-114
+                         (22: |eps_tot - eps|, 23: |sig - lambda|); 0 for gas voxels */
+        memcpy(out, field == 22 ? J->err_e_v : J->err_s_v, sizeof(double) * V);
+        return 1;
     case F_SCHMID:
         for (c = 0; c < 60; c++)
             for (p = 0; p < V; p++) out[c * V + p] = J->Schm_gr[p][c / 5][c % 5];

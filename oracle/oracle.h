@@ -102,6 +102,20 @@ void orc_harden_voxel(double crss[24], double *gamacum, double gamtot[12],
 /* Latent-hardening matrix exactly as PlasticInit builds it (init.c:283-381). */
 void orc_hard_matrix(double self, double coplane, double cross, double glissile,
                      double hirth, double lomer, double H[144]);
+/* update_orient for one voxel (constitutive.c:188-229): tran updated in place. */
+void orc_update_orient_voxel(double tran[9], const double velgrad[9], const double udot_a[9],
+                             const double *bn, const double *bb, const double *gamdot, int nsys, double dt);
+/* update_schmid for one voxel (kinematics.c:436-456): out = nsys x 5. */
+void orc_update_schmid_voxel(const double tran[9], const double *schm_xt, int nsys, double *out);
+/* get_smacro's mixed-BC strain correction (evolution.c:199-211). */
+void orc_bc_correction(const double S0[81], const int vflag[6], const int sflag[6],
+                       const double Scauchy[9], const double SigAvg[9], double dD[9]);
+/* Err_e, Err_s from the voxel sums (evolution.c:410-423). */
+void orc_err_normalise(double err_e_sum, double err_s_sum, double wgt, double e_vm, double s_vm,
+                       double out[2]);
+/* rho_rss, rho_crss, work_p of one voxel (evolution.c:509-529). */
+void orc_rho_work_voxel(const double sig[9], const double *schm_gr, const double *crss,
+                        const double *gamtot, int nsys, double out[3]);
 /* Namelist parse only (io.c:54-122); returns number of keys with status 1,
  * and fills status[26] in the reference's key order. */
 int orc_parse_namelist(const char *path, int status[26], double *vals_out, int nvals);

@@ -60,6 +60,12 @@ def lib():
         L.orc_harden_voxel.argtypes = [D, D, D, D, C.c_int, D, D, D, C.c_double]
         L.orc_hard_matrix.argtypes = [C.c_double] * 6 + [D]
         L.orc_parse_namelist.argtypes = [C.c_char_p, C.POINTER(C.c_int), D, C.c_int]
+        L.orc_update_orient_voxel.argtypes = [D, D, D, D, D, D, C.c_int, C.c_double]
+        L.orc_update_schmid_voxel.argtypes = [D, D, C.c_int, D]
+        IP = C.POINTER(C.c_int)
+        L.orc_bc_correction.argtypes = [D, IP, IP, D, D, D]
+        L.orc_err_normalise.argtypes = [C.c_double] * 5 + [D]
+        L.orc_rho_work_voxel.argtypes = [D, D, D, D, C.c_int, D]
         _lib = L
     return _lib
 
@@ -70,11 +76,11 @@ def ptr(a: np.ndarray):
 
 
 FIELD_NCOMP = {0: 6, 1: 6, 2: 6, 3: 6, 4: 9, 5: 12, 6: 12, 7: 12, 8: 12, 9: 12, 10: 9, 11: 1, 12: 1,
-               13: 1, 14: 1, 15: 1, 16: 1, 17: 9, 18: 60, 19: 1, 20: 1, 21: 3}
+               13: 1, 14: 1, 15: 1, 16: 1, 17: 9, 18: 60, 19: 1, 20: 1, 21: 3, 22: 1, 23: 1}
 FIELD_IDS = {"sig": 0, "eps": 1, "edot": 2, "els": 3, "disgrad": 4, "crss": 5, "crss_bwd": 6,
              "gamtot": 7, "gamdot": 8, "taux": 9, "tranmat": 10, "gamacum": 11, "svm": 12, "evm": 13,
              "work_p": 14, "rho_rss": 15, "rho_crss": 16, "velgrad": 17, "schmid": 18,
-             "nr_iters": 19, "work_e": 20, "new_position": 21}
+             "nr_iters": 19, "work_e": 20, "new_position": 21, "err_e_v": 22, "err_s_v": 23}
 
 
 def report_dict(r: OrcReport) -> dict:

@@ -500,3 +500,236 @@ def test_isotropic_elastic_polycrystal_hooke(tmp_path):
     assert np.isclose(dg[1, 1], -nu * e11, rtol=1e-7)
     s = j.field("sig")
     assert np.abs(s - s[:, :1]).max() < 1e-9 * np.abs(s).max()
+
+
+# ---------------------------------------------------------------------------
+# lattice spin (update_orient, constitutive.c:171-229), update_schmid
+# (kinematics.c:426-459), the mixed-BC correction (evolution.c:199-211), the
+# error normalisation (evolution.c:410-423) and the rho/work diagnostics
+# (evolution.c:509-529)
+# ---------------------------------------------------------------------------
+
+def unit_slip():
+    n = np.array([s[0] for s in inputs.FCC_SLIP], float)
+    b = np.array([s[1] for s in inputs.FCC_SLIP], float)
+    return n / np.linalg.norm(n, axis=1)[:, None], b / np.linalg.norm(b, axis=1)[:, None]
+
+
+def update_orient(tran, velgrad, udot_a, gamdot, dt):
+    bn, bb = unit_slip()
+    t = np.ascontiguousarray(tran, dtype=float).ravel().copy()
+    L().orc_update_orient_voxel(P(t), P(np.ascontiguousarray(velgrad, float).ravel().copy()),
+                                P(np.ascontiguousarray(udot_a, float).ravel().copy()), P(bn.ravel().copy()),
+                                P(bb.ravel().copy()), P(np.asarray(gamdot, float).copy()), 12, dt)
+    return t.reshape(3, 3)
+
+
+def rot_axis_angle(axis, angle):
+    # Rodrigues: rotation by `angle` about the unit vector `axis`
+    k = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+def test_lattice_spin_single_slip_closed_form():
+    # one active system: plastic spin W_p = gd/2 (b_s (x) n_s - n_s (x) b_s)
+    # with sample-frame vectors b_s = R b, n_s = R n, so the lattice turns by
+    # -W_p dt: angle gd dt / 2 about b_s x n_s
+    bn, bb = unit_slip()
+    R = euler(35, 60, 125).T          # TranMat = crystal -> sample
+    for s, gd, dt in [(0, 4e-3, 1.0), (7, -2.5e-3, 0.5), (11, 1e-2, 0.2)]:
+        g = np.zeros(12)
+        g[s] = gd
+        R1 = update_orient(R, np.zeros((3, 3)), np.zeros((3, 3)), g, dt)
+        bs, ns = R @ bb[s], R @ bn[s]
+        ref = rot_axis_angle(np.cross(bs, ns), gd * dt / 2) @ R
+        assert np.abs(R1 - ref).max() < 1e-14, s
+
+
+def test_lattice_spin_rigid_rotation_and_applied_spin():
+    # no slip: the lattice follows the local spin skew(VelGrad) exactly
+    # (the symmetric part of VelGrad does not rotate it), plus Udot_a
+    R = euler(200, 30, 75).T
+    w = np.array([1.5e-3, -2e-3, 0.7e-3])
+    W = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    D = np.array([[1e-3, 2e-4, 0], [2e-4, -5e-4, 1e-4], [0, 1e-4, -5e-4]])
+    dt = 2.0
+    R1 = update_orient(R, D + W, np.zeros((3, 3)), np.zeros(12), dt)
+    ref = rot_axis_angle(w, np.linalg.norm(w) * dt) @ R
+    assert np.abs(R1 - ref).max() < 1e-14
+    # the same spin given as the applied spin instead
+    R2 = update_orient(R, D, W, np.zeros(12), dt)
+    assert np.abs(R2 - ref).max() < 1e-14
+    # spins add: half local, half applied
+    R3 = update_orient(R, D + W / 2, W / 2, np.zeros(12), dt)
+    assert np.abs(R3 - ref).max() < 1e-14
+
+
+def test_lattice_spin_all_systems_against_expm():
+    from scipy.linalg import expm
+    bn, bb = unit_slip()
+    rng = np.random.default_rng(21)
+    R = euler(10, 80, 300).T
+    gd = rng.standard_normal(12) * 3e-3
+    Lv = rng.standard_normal((3, 3)) * 2e-3
+    dt = 0.7
+    R1 = update_orient(R, Lv, np.zeros((3, 3)), gd, dt)
+    Lp = np.einsum("s,si,sj->ij", gd, bb @ R.T, bn @ R.T)
+    W = 0.5 * (Lv - Lv.T) - 0.5 * (Lp - Lp.T)
+    assert np.abs(R1 - expm(W * dt) @ R).max() < 1e-14
+    assert np.abs(R1 @ R1.T - np.eye(3)).max() < 1e-15
+
+
+def test_update_schmid_rotates_crystal_schmid():
+    # P_sample = R P_xtal R^T: symmetric, traceless, and P_sample : sigma =
+    # P_xtal : (R^T sigma R) for any sigma; R = TranMat (crystal -> sample)
+    B = basis()
+    a = euler(73, 41, 199)              # sample -> crystal
+    R = a.T
+    px = fcc_schmid()                   # crystal frame
+    out = np.zeros(60)
+    L().orc_update_schmid_voxel(P(R.ravel().copy()), P(px.ravel().copy()), 12, P(out))
+    Ps = np.einsum("sa,aij->sij", out.reshape(12, 5), B[:5])
+    Px = np.einsum("sa,aij->sij", px, B[:5])
+    assert np.abs(Ps - Ps.transpose(0, 2, 1)).max() < 1e-15
+    assert np.abs(np.einsum("sii->s", Ps)).max() < 1e-15
+    rng = np.random.default_rng(22)
+    for _ in range(3):
+        sig = rng.standard_normal((3, 3))
+        sig = sig + sig.T
+        lhs = np.einsum("sij,ij->s", Ps, sig)
+        rhs = np.einsum("sij,ij->s", Px, R.T @ sig @ R)
+        assert np.abs(lhs - rhs).max() < 1e-13
+    # the Euler -> TranMat chain of init.c:430-434 gives the same tensors as
+    # building the Schmid tensor from the sample-frame vectors a^T n, a^T b
+    assert np.abs(out.reshape(12, 5) - fcc_schmid(a)).max() < 1e-14
+
+
+def iso_compliance(E, nu):
+    d = np.eye(3)
+    return ((1 + nu) / E * 0.5 * (np.einsum("ik,jl->ijkl", d, d) + np.einsum("il,jk->ijkl", d, d))
+            - nu / E * np.einsum("ij,kl->ijkl", d, d))
+
+
+def bc_correction(S0, vflag, sflag, Sc, Sa, fill=7.0):
+    dD = np.full(9, fill)
+    L().orc_bc_correction(P(S0.ravel().copy()), (C.c_int * 6)(*vflag), (C.c_int * 6)(*sflag),
+                          P(np.asarray(Sc, float).ravel().copy()), P(np.asarray(Sa, float).ravel().copy()), P(dD))
+    return dD.reshape(3, 3)
+
+
+def test_bc_correction_hand_solved():
+    E, nu = 120000.0, 0.3
+    mu = E / (2 * (1 + nu))
+    S0 = iso_compliance(E, nu)
+    rng = np.random.default_rng(23)
+    sa = rng.standard_normal((3, 3)) * 10
+    sa = sa + sa.T
+    upper = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    lower = [(1, 0), (2, 0), (2, 1)]
+    # x tension, lateral faces stress free: only D22 and D33 are corrected,
+    # by the isotropic compliance of the lateral misfits (Hooke by hand)
+    d = bc_correction(S0, (1, 0, 0, 1, 1, 1), (0, 1, 1, 0, 0, 0), np.zeros((3, 3)), sa)
+    ref = {(1, 1): (-sa[1, 1] + nu * sa[2, 2]) / E, (2, 2): (nu * sa[1, 1] - sa[2, 2]) / E}
+    for ij in upper:
+        assert np.isclose(d[ij], ref.get(ij, 0.0), rtol=1e-14, atol=1e-22), ij
+    assert all(d[ij] == 7.0 for ij in lower)    # only the upper triangle is written
+    # shear stress control on 23: D_23 = (S*_23 - <s>_23) / (4 mu), half the
+    # tensor shear strain, written to (1,2) only; (2,1) is left alone
+    sc = np.zeros((3, 3))
+    sc[1, 2] = sc[2, 1] = 5.0
+    d = bc_correction(S0, (1, 1, 1, 0, 1, 1), (0, 0, 0, 1, 0, 0), sc, sa)
+    for ij in upper:
+        want = (5.0 - sa[1, 2]) / (4 * mu) if ij == (1, 2) else 0.0
+        assert np.isclose(d[ij], want, rtol=1e-14, atol=1e-22), ij
+    assert all(d[ij] == 7.0 for ij in lower)
+    # stress flags decide which misfits count: with the 13 and 12 misfits
+    # flagged but 23 not, nothing is corrected (isotropic S0 does not couple them)
+    d = bc_correction(S0, (1, 1, 1, 0, 1, 1), (0, 0, 0, 0, 1, 1), sc, sa)
+    assert all(d[ij] == 0.0 for ij in upper)
+    # creep (everything stress controlled)
+    sc = rng.standard_normal((3, 3))
+    sc = sc + sc.T
+    m = sc - sa
+    d = bc_correction(S0, (0,) * 6, (1,) * 6, sc, sa)
+    ref = {(0, 0): (m[0, 0] - nu * (m[1, 1] + m[2, 2])) / E, (1, 1): (m[1, 1] - nu * (m[0, 0] + m[2, 2])) / E,
+           (2, 2): (m[2, 2] - nu * (m[0, 0] + m[1, 1])) / E, (1, 2): m[1, 2] / (4 * mu),
+           (0, 2): m[0, 2] / (4 * mu), (0, 1): m[0, 1] / (4 * mu)}
+    for ij in upper:
+        assert np.isclose(d[ij], ref[ij], rtol=1e-13), ij
+
+
+def vm_ref(t):
+    # VonMises of evolution.c:243-262: sqrt(2/3 dev(sym t) : dev(sym t))
+    s = 0.5 * (t + t.T)
+    dv = s - np.trace(s) / 3 * np.eye(3)
+    return math.sqrt(2.0 / 3.0 * np.sum(dv * dv))
+
+
+def voigt_to_t(f, v):
+    i = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    t = np.zeros((3, 3))
+    for c, (a, b) in enumerate(i):
+        t[a, b] = t[b, a] = f[c, v]
+    return t
+
+
+@pytest.mark.parametrize("dt,divided", [(1.0, True), (5e-3, False)])
+def test_error_normalisation_from_fields(tmp_path, dt, divided):
+    # one FFT iteration per increment (IterMax 1): Err_s = (1/V) sum_v
+    # |sig_new - lambda| / s_vm(<sig_new>), Err_e = (1/V) sum_v
+    # |eps_tot - eps| / e_vm with e_vm = d_vm dt at step 1 (init.c:536) and
+    # the previous increment's macro strain after that, applied only when
+    # |e_vm| > 1e-5 (dt = 5e-3 gives e_vm = 5e-6: not divided at step 1)
+    c = inputs.Case(dims=(4, 4, 2), n_grains=3, seed=4, n_steps=2, iter_max=1, time_step=dt)
+    d = str(tmp_path)
+    inputs.write_case(d, c)
+    j = po.Job(os.path.join(d, "input"), d, threads=2)
+    V = j.nvox
+    e_prev = vm_ref(np.diag(c.velgrad[:3])) * dt
+    for step in range(2):
+        s0 = j.field("sig")
+        r = j.increment()
+        s1 = j.field("sig")
+        ev, es = j.field("err_e_v")[0], j.field("err_s_v")[0]
+        # the per-voxel stress partial is the Frobenius norm of the stress change
+        dn = np.array([np.linalg.norm(voigt_to_t(s1, v) - voigt_to_t(s0, v)) for v in range(V)])
+        assert np.allclose(es, dn, rtol=1e-12, atol=1e-300)
+        sbar = np.array([[s1[k].mean() for k in range(6)]])
+        svm = math.sqrt(1.5) * np.linalg.norm(voigt_to_t(sbar.T, 0) - np.trace(voigt_to_t(sbar.T, 0)) / 3 * np.eye(3))
+        assert np.isclose(r["s_vm"], svm, rtol=1e-12)
+        assert r["iter"] == 1
+        assert np.isclose(r["err_s"][0], es.sum() / V / svm, rtol=1e-12)
+        want_e = ev.sum() / V / e_prev if (abs(e_prev) > 1e-5) else ev.sum() / V
+        if step == 0:
+            assert (abs(e_prev) > 1e-5) == divided
+        assert np.isclose(r["err_e"][0], want_e, rtol=1e-12), step
+        e_prev = vm_ref(r["disgrad_avg"])
+        assert np.isclose(r["e_vm"], e_prev, rtol=1e-14)
+    j.close()
+
+
+def test_rho_work_diagnostics_from_fields(tmp_path):
+    # rho_rss = sum_s (tau_s/c)^2, rho_crss = sum_s (crss_s/c)^2,
+    # work_p = sum_s |gammatotal_s tau_s|, c = 0.35 * 47.565 * 0.256, with
+    # tau_s = P_s : sigma over the full tensors (evolution.c:509-529); the
+    # forward CRSS counts (tau0f != tau0b tells it from the backward one)
+    c = inputs.Case(dims=(4, 4, 4), n_grains=3, seed=6, n_steps=3, material=inputs.Material(tau0f=16.0, tau0b=21.0))
+    j, _ = run_oracle(tmp_path, c, 3)
+    V = j.nvox
+    B = basis()
+    sig, sch, cr, gt = j.field("sig"), j.field("schmid"), j.field("crss"), j.field("gamtot")
+    k = 0.35 * 47.565 * 0.256
+    rr, rc, wp = np.zeros(V), np.zeros(V), np.zeros(V)
+    for v in range(V):
+        S = voigt_to_t(sig, v)
+        Ps = np.einsum("sa,aij->sij", sch[:, v].reshape(12, 5), B[:5])
+        tau = np.einsum("sij,ij->s", Ps, S)
+        rr[v] = np.sum((tau / k) ** 2)
+        rc[v] = np.sum((cr[:, v] / k) ** 2)
+        wp[v] = np.sum(np.abs(gt[:, v] * tau))
+    assert np.abs(gt).max() > 0 and np.abs(cr - 16.0).max() > 0    # hardening has run
+    for name, ref in (("rho_rss", rr), ("rho_crss", rc), ("work_p", wp)):
+        got = j.field(name)[0]
+        assert np.allclose(got, ref, rtol=1e-11, atol=0), name
+    j.close()
