@@ -124,6 +124,14 @@ int voce_server_poll(voce_server *s, voce_result *out, int max);
 /* out[0..9]: registered ids, resident bytes, workspace pool bytes, pool bytes
  * in use, in flight, queued, completed, audits, steps, FFT iterations launched */
 int voce_server_info(const voce_server *s, long out[10]);
+/* Failure-handling test hook (no reference counterpart): with kind
+ * VOCE_FAULT_SHADOW_ALLOC, the `after`-th next allocation of an audit's FP64
+ * shadow reports VOCE_ERR_NOMEM (0 switches the hook off).  The failing
+ * voce_server_step returns the error with the request back in the queue in
+ * its place and nothing of it held; increments admitted before it in the
+ * same round proceed. */
+#define VOCE_FAULT_SHADOW_ALLOC 1
+int voce_server_inject_fault(voce_server *s, int kind, int after);
 /* The id's context (fields of its resident state; owned by the server). */
 voce_ctx *voce_server_rve(voce_server *s, int id);
 void voce_server_destroy(voce_server *s);

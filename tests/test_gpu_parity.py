@@ -33,9 +33,13 @@ def compare(tmp_path, case, n, grain=None, euler=None, fields=FIELDS, macro_tol=
             assert rel(g[k][key], o[k][key]) < macro_tol, key
         for key in ("e_vm", "evmp", "d_vm", "dvmp", "s_vm", "s_vm1"):
             assert abs(g[k][key] - o[k][key]) <= macro_tol * max(abs(o[k][key]), 1e-300), key
-        # err.out rows: errors are differences of nearby stresses, so looser
-        assert rel(g[k]["err_s"], o[k]["err_s"]) < 1e-6
-        assert rel(g[k]["err_e"], o[k]["err_e"]) < 1e-6
+        # err.out rows: errors are differences of nearby stresses, so looser;
+        # an increment that converges in one iteration leaves rows of pure
+        # round-off (~1e-16), compared against an absolute floor of 1e-12
+        # (far below any tolerance Err a namelist states)
+        for key in ("err_s", "err_e"):
+            d = np.abs(g[k][key] - o[k][key]).max() if len(o[k][key]) else 0.0
+            assert d <= 1e-6 * np.abs(o[k][key]).max(initial=0.0) + 1e-12, key
     for f in fields:
         r = rel(gf[f], of[f])
         say("field", f, "rel", r)

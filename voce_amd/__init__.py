@@ -94,7 +94,8 @@ EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_fi
            "voce_batch_size", "voce_batch_member", "voce_batch_info", "voce_batch_destroy",
            "voce_set_velgrad", "voce_voronoi", "voce_load_case_voronoi",
            "voce_server_create", "voce_server_add_rve", "voce_server_submit", "voce_server_step",
-           "voce_server_poll", "voce_server_info", "voce_server_rve", "voce_server_destroy"]
+           "voce_server_poll", "voce_server_info", "voce_server_rve", "voce_server_destroy",
+           "voce_server_inject_fault"]
 
 PROF_KERNELS = ["plane_fwd", "xgreen", "plane_inv", "nr", "fin_iter"]
 
@@ -150,6 +151,7 @@ def load_library(path: str = LIB_PATH):
     lib.voce_server_info.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
     lib.voce_server_rve.argtypes = [C.c_void_p, C.c_int]
     lib.voce_server_rve.restype = C.c_void_p
+    lib.voce_server_inject_fault.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.voce_server_destroy.argtypes = [C.c_void_p]
     lib.voce_server_destroy.restype = None
     _lib = lib
@@ -482,6 +484,13 @@ class Server:
         out = (C.c_long * 10)()
         load_library().voce_server_info(self._h, out)
         return dict(zip(self.INFO_KEYS, list(out)))
+
+    def inject_fault(self, kind=1, after=1):
+        """Failure-handling test hook: the `after`-th next audit-shadow
+        allocation reports VOCE_ERR_NOMEM (kind 1, VOCE_FAULT_SHADOW_ALLOC)."""
+        rc = load_library().voce_server_inject_fault(self._h, int(kind), int(after))
+        if rc:
+            raise VoceError(rc, "voce_server_inject_fault")
 
     def rve(self, rid) -> Solver:
         hm = load_library().voce_server_rve(self._h, int(rid))

@@ -142,6 +142,7 @@ struct Shadow {
 
 struct Inflight {
     int row;
+    long seq;                // admission sequence number (pairs a row with its shadow)
     Request req;
     double t_admit;
     int shadow;              // index into shadows, or -1
@@ -172,7 +173,9 @@ struct voce_server {
     std::deque<Request> queue;
     std::vector<Inflight> inflight;
     std::vector<voce_result> done;
-    std::map<long, PendingAudit> audits;    // by request tag
+    std::map<long, PendingAudit> audits;    // by admission sequence number (tags need not be unique)
+    long next_seq = 0;
+    int fail_shadow_alloc = 0;              // test hook: fail the n-th next shadow allocation (0: never)
     std::map<std::tuple<int, int, int>, voce_tune_choice> tuned;
     // device scratch: maps, poll indices and flags; pinned staging
     int2 *d_maps = nullptr;
@@ -381,6 +384,56 @@ extern "C" int voce_server_submit(voce_server *s, int id, const double L[9], int
 
 // FP64 shadow of row `src` in shadow slot k: own evolving state and
 // workspace, the source's constant tables, a copy of its state
+// bytes of a shadow arena for id `src`: evolving arrays, P6, spectra, sums,
+// partials, MacroState, logs, twiddles (FP64)
+static size_t shadow_bytes(voce_server *s, int src)
+{
+    voce_ctx *c = s->ctx[src];
+    const RveDev &d0 = c->d;
+    const long V = d0.V;
+    const size_t db = sizeof(double), cb = 16;
+    const int two = d0.CRSSB != d0.CRSS;
+    RveDev d = d0;
+    d.fft_f32 = 0;
+    voce_spec_layout(&d);
+    const size_t nspec = d.nspec;
+    size_t need = 0;
+    const size_t sz_state[11] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
+                                 9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db};
+    for (size_t b : sz_state) need += al256(b);
+    need += al256(6 * V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(V) * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) +
+            al256((size_t)d0.nblk_vox * 16 * db);
+    need += al256(sizeof(MacroState)) + al256(3 * (size_t)c->log_cap * db);
+    need += al256(d0.nx * cb) + al256(d0.ny * cb) + al256(d0.nz * cb);
+    return need;
+}
+
+// device memory of shadow slot k, large enough for id `src` (the only step
+// of an audit's set-up that can run out of memory; done before the request
+// takes anything else)
+static int shadow_reserve(voce_server *s, int k, int src)
+{
+    Shadow &sh = s->shadows[k];
+    const size_t need = shadow_bytes(s, src);
+    if (s->fail_shadow_alloc > 0 && --s->fail_shadow_alloc == 0) return VOCE_ERR_NOMEM;
+    if (sh.arena && sh.bytes < need) {
+        (void)hipFree(sh.arena);
+        sh.arena = nullptr;
+        sh.bytes = 0;
+    }
+    if (!sh.arena) {
+        if (hipMalloc((void **)&sh.arena, need) != hipSuccess) {
+            sh.arena = nullptr;
+            return VOCE_ERR_NOMEM;
+        }
+        sh.bytes = need;
+    }
+    return 0;
+}
+
+// FP64 shadow of row `src` in shadow slot k (reserved by shadow_reserve):
+// own evolving state and workspace, the source's constant tables, a copy of
+// its state
 static int shadow_start(voce_server *s, int k, int src, const double L[9])
 {
     Shadow &sh = s->shadows[k];
@@ -394,28 +447,14 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     d.fft_f32 = 0;
     voce_spec_layout(&d);   // FP64 spectrum layout
     const size_t nspec = d.nspec;
-    size_t need = 0;
     const size_t sz_state[11] = {6 * V * db, 6 * V * db, 5 * V * db, 5 * V * db, 12 * V * db, two ? 12 * V * db : 0,
                                  9 * V * db, 3 * V * db, 3 * V * db, V * db, 12 * V * db};
-    for (size_t b : sz_state) need += al256(b);
-    need += al256(6 * V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(V) * db) + al256(6 * nspec * cb) + al256(3 * nspec * cb) +
-            al256((size_t)d0.nblk_vox * 16 * db);
-    need += al256(sizeof(MacroState)) + al256(3 * (size_t)c->log_cap * db);
-    need += al256(d0.nx * cb) + al256(d0.ny * cb) + al256(d0.nz * cb);
-    if (sh.arena && sh.bytes < need) {
-        (void)hipFree(sh.arena);
-        sh.arena = nullptr;
-    }
-    if (!sh.arena) {
-        if (hipMalloc((void **)&sh.arena, need) != hipSuccess) {
-            sh.arena = nullptr;
-            return VOCE_ERR_NOMEM;
-        }
-        sh.bytes = need;
-    }
+    if (!sh.arena || sh.bytes < shadow_bytes(s, src)) return VOCE_ERR_ARG;
     sh.src = src;
     sh.V = V;
-    d.GAMDOT = d.TAUX = d.SVM = d.EVM = d.WORKP = d.RHOR = d.RHOC = nullptr;
+    // no diagnostics: the shadow must not write into the id's arrays (the
+    // fluctuation sums FLU are updated whenever the pointer is set)
+    d.GAMDOT = d.TAUX = d.SVM = d.EVM = d.WORKP = d.RHOR = d.RHOC = d.FLU = nullptr;
     char *p = sh.arena;
     double *dst[11];
     const double *srcp[11] = {d0.E6, d0.S6, d0.EP5, d0.ED5, d0.CRSS, two ? d0.CRSSB : nullptr, d0.R, d0.WG, d0.WF,
@@ -643,8 +682,12 @@ static int server_step(voce_server *s)
 {
     int rc = 0;
     ++s->steps;
-    // 1. admission
+    // 1. admission.  A failure part way (no memory for an audit's shadow, a
+    // device error) puts the request back where it was, releases what it had
+    // taken, and still starts the increments admitted before it, so the
+    // server stays consistent and the request can be retried.
     std::vector<int> started;
+    int adm_rc = 0;
     {
         std::deque<Request> keep;
         bool blocked = false;
@@ -664,6 +707,25 @@ static int server_step(voce_server *s)
                     }
                 if (sk < 0) ok = false;
             }
+            bool took_ws = false;
+            auto fail = [&](int code) {
+                if (took_ws) drop_ws(s, row, c->d);
+                s->htab[row] = c->d;
+                s->busy[row] = 0;
+                keep.push_back(r);   // back in arrival order, ahead of the rest
+                while (!s->queue.empty()) {
+                    keep.push_back(s->queue.front());
+                    s->queue.pop_front();
+                }
+                adm_rc = code;
+            };
+            if (ok && r.audit) {
+                const int e = shadow_reserve(s, sk, row);
+                if (e) {
+                    fail(e);
+                    break;
+                }
+            }
             if (ok && !take_ws(s, row, c->d)) {
                 ok = false;
                 blocked = true;   // keep arrival order under memory pressure
@@ -672,42 +734,75 @@ static int server_step(voce_server *s)
                 keep.push_back(r);
                 continue;
             }
+            took_ws = true;
             // the id's next increment: velocity gradient, workspace pointers
             memcpy(s->h_vg + 9 * row, r.L, 9 * sizeof(double));
-            SCK(hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_next), s->h_vg + 9 * row, 9 * sizeof(double),
-                               hipMemcpyHostToDevice, s->stream));
-            SCK(hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_pending), s->h_one + 1, sizeof(int),
-                               hipMemcpyHostToDevice, s->stream));
+            if (hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_next), s->h_vg + 9 * row, 9 * sizeof(double),
+                               hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+                hipMemcpyAsync((char *)c->d.ms + offsetof(MacroState, vg_pending), s->h_one + 1, sizeof(int),
+                               hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+                fail(VOCE_ERR_DEVICE);
+                break;
+            }
             s->htab[row] = c->d;
-            if ((rc = upload_row(s, row))) return rc;
-            s->busy[row] = 1;
+            if (upload_row(s, row)) {
+                fail(VOCE_ERR_DEVICE);
+                break;
+            }
             Inflight f;
             f.row = row;
+            f.seq = s->next_seq++;
             f.req = r;
             f.t_admit = mono_now();
             f.shadow = -1;
-            started.push_back(row);
             if (r.audit) {
                 // the shadow copies the state before the increment starts
-                if ((rc = shadow_start(s, sk, row, r.L))) return rc;
+                const int e = shadow_start(s, sk, row, r.L);
+                if (e) {
+                    fail(e);
+                    break;
+                }
                 s->busy[s->shadows[sk].row] = 1;
                 f.shadow = sk;
-                started.push_back(s->shadows[sk].row);
                 Inflight fs;
                 fs.row = s->shadows[sk].row;
+                fs.seq = f.seq;
                 fs.req = r;
                 fs.t_admit = f.t_admit;
                 fs.shadow = sk;
                 s->inflight.push_back(fs);
+                started.push_back(fs.row);
                 PendingAudit pa;
                 memset(&pa.res, 0, sizeof(pa.res));
-                s->audits[r.tag] = pa;
+                s->audits[f.seq] = pa;
                 ++s->n_audits;
             }
+            s->busy[row] = 1;
+            started.push_back(row);
             s->inflight.push_back(f);
             s->set_changed = true;
         }
         s->queue.swap(keep);
+    }
+    if (adm_rc) {
+        // start what was admitted before the failure, then report it
+        if (!started.empty()) {
+            std::vector<int> rows;
+            for (auto &f : s->inflight) rows.push_back(f.row);
+            if (ensure_scratch(s, map_need(s, rows), rows.size())) return adm_rc;
+            int2 *hb = s->h_maps;
+            for (size_t i = 0; i < started.size(); ++i) hb[i] = make_int2(started[i], 0);
+            LaunchSet B;
+            memset(&B, 0, sizeof(B));
+            B.tab = s->dtab;
+            B.map_one = s->d_maps;
+            B.n_one = (int)started.size();
+            if (hipMemcpyAsync(s->d_maps, hb, started.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream) ==
+                hipSuccess)
+                (void)voce_launch_begin_increment(&B, s->stream);
+            (void)hipStreamSynchronize(s->stream);
+        }
+        return adm_rc;
     }
     if (s->inflight.empty()) return 0;
     std::vector<int> rows;
@@ -790,7 +885,7 @@ static int server_step(voce_server *s)
             sh.busy = false;
             s->busy[f.row] = 0;
             s->htab[f.row].ms = nullptr;
-            PendingAudit &pa = s->audits[f.req.tag];
+            PendingAudit &pa = s->audits[f.seq];
             pa.shadow_ms = M;
             pa.shadow_done = true;
         } else {
@@ -812,7 +907,7 @@ static int server_step(voce_server *s)
             res.t_admit = f.t_admit;
             res.t_done = t_done;
             if (f.req.audit) {
-                PendingAudit &pa = s->audits[f.req.tag];
+                PendingAudit &pa = s->audits[f.seq];
                 pa.res = res;
                 pa.main_done = true;
             } else {
@@ -821,7 +916,7 @@ static int server_step(voce_server *s)
             }
         }
         if (f.req.audit) {
-            auto it = s->audits.find(f.req.tag);
+            auto it = s->audits.find(f.seq);
             if (it != s->audits.end() && it->second.main_done && it->second.shadow_done) {
                 voce_result res = it->second.res;
                 res.audit_sig_rel = sig_rel(res.rep.sig_avg, it->second.shadow_ms.SigAvg);
@@ -864,6 +959,13 @@ extern "C" int voce_server_info(const voce_server *s, long out[10])
     out[7] = s->n_audits;
     out[8] = s->steps;
     out[9] = s->iters;
+    return 0;
+}
+
+extern "C" int voce_server_inject_fault(voce_server *s, int kind, int after)
+{
+    if (!s || kind != VOCE_FAULT_SHADOW_ALLOC || after < 0) return VOCE_ERR_ARG;
+    s->fail_shadow_alloc = after;
     return 0;
 }
 
