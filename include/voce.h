@@ -53,6 +53,7 @@ extern "C" {
 #define VOCE_ERR_DEVICE (-4)
 #define VOCE_ERR_NOMEM (-5)
 #define VOCE_ERR_NOT_LOADED (-6)
+#define VOCE_ERR_SCRATCH (-7)   /* RVE too large for voce_increment_noalloc's workspace */
 
 /* precision modes */
 #define VOCE_FP64 0   /* everything FP64 (drop-in path) */
@@ -163,6 +164,21 @@ int voce_ctx_create(const voce_setup *setup, voce_ctx **out);
  * strain, lattice rotation from step 2, hardening from step 3).  All on
  * the device; the report is copied back at the end. */
 int voce_increment(voce_ctx *ctx, voce_incr_report *report);
+
+/* voce_increment without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory during a solve (tight serving
+ * loops, code that runs between HIP graph captures, harnesses that forbid
+ * hipMalloc): the increment's workspace (pre-update stress, crystal-frame
+ * K_c, spectra, skew sums, block partials) comes from one static device
+ * array of VOCE_NOALLOC_MAX_BYTES per device instead of the context's own,
+ * and the call makes no hipMalloc/hipFree (it only needs the context, which
+ * voce_ctx_create has built).  Results are bit-identical to voce_increment.
+ * Limits: a context of voce_ctx_create (not a batch member or server id)
+ * with IterMax >= 1, whose workspace fits the array (one 64^3 RVE in FP64,
+ * about 73 MiB; larger RVEs return VOCE_ERR_SCRATCH).  Every call on one
+ * device shares that device's array: calls that overlap on a device race. */
+#define VOCE_NOALLOC_MAX_BYTES (96u << 20)
+int voce_increment_noalloc(voce_ctx *ctx, voce_incr_report *report);
 
 /* Copy one field to host memory (bytes >= ncomp*V*sizeof(double)). */
 int voce_get_field(voce_ctx *ctx, int field, double *host, size_t bytes);

@@ -84,7 +84,7 @@ class Case(C.Structure):
 
 
 # every symbol include/voce.h declares
-EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_get_field",
+EXPORTS = ["voce_field_ncomp", "voce_ctx_create", "voce_increment", "voce_increment_noalloc", "voce_get_field",
            "voce_set_diagnostics", "voce_ctx_nvox", "voce_ctx_destroy", "voce_strerror",
            "voce_load_case", "voce_case_free", "voce_write_fields",
            # include/voce_prof.h
@@ -115,6 +115,7 @@ def load_library(path: str = LIB_PATH):
     lib.voce_field_ncomp.argtypes = [C.c_int]
     lib.voce_ctx_create.argtypes = [C.POINTER(Setup), C.POINTER(C.c_void_p)]
     lib.voce_increment.argtypes = [C.c_void_p, C.POINTER(Report)]
+    lib.voce_increment_noalloc.argtypes = [C.c_void_p, C.POINTER(Report)]
     lib.voce_get_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]
     lib.voce_set_diagnostics.argtypes = [C.c_void_p, C.c_int]
     lib.voce_ctx_nvox.restype = C.c_long

@@ -37,7 +37,8 @@ def build(verbose_resources=False):
     objs = []
     jobs = []
     for src, flags in (("voce_kernels.hip", extra), ("voce_ctx.cpp", []), ("voce_tune.cpp", []),
-                       ("voce_batch.cpp", []), ("voce_serve.cpp", []), ("voce_gen.cpp", [])):
+                       ("voce_batch.cpp", []), ("voce_serve.cpp", []), ("voce_gen.cpp", []),
+                       ("voce_noalloc.hip", [])):
         o = os.path.join(obj, os.path.splitext(src)[0] + ".o")
         objs.append(o)
         jobs.append([HIPCC] + HIP_FLAGS + flags + ["-c", os.path.join(CSRC, src), "-o", o])

@@ -244,6 +244,31 @@ int voce_dev_alloc(voce_ctx *c, void **p, size_t bytes)
     return 0;
 }
 
+// Per-increment workspace of one RVE (pre-update stress, crystal-frame K_c,
+// spectra, skew sums, block partials), each part 256-byte aligned.
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t voce_ws_bytes(const RveDev &d)
+{
+    const size_t cb = d.fft_f32 ? 8 : 16, db = sizeof(double);
+    return al256(6 * d.V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db) + al256(6 * d.nspec * cb) +
+           al256(3 * d.nspec * cb) + al256((size_t)d.nblk_vox * 16 * db);
+}
+
+void voce_ws_carve(RveDev &d, char *p)
+{
+    const size_t cb = d.fft_f32 ? 8 : 16, db = sizeof(double);
+    d.P6 = (double *)p;
+    p += al256(6 * d.V * db);
+    d.KCV = (double *)p;
+    p += al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db);
+    d.spec = p;
+    p += al256(6 * d.nspec * cb);
+    d.usum = p;
+    p += al256(3 * d.nspec * cb);
+    d.part = (double *)p;
+}
+
 // Push the host descriptor to its device copy and refresh the launch set.
 int voce_sync_tab(voce_ctx *c)
 {
@@ -302,6 +327,7 @@ extern "C" const char *voce_strerror(int code)
     case VOCE_ERR_DEVICE: return "HIP runtime error";
     case VOCE_ERR_NOMEM: return "device memory allocation failed";
     case VOCE_ERR_NOT_LOADED: return "HIP library not loaded";
+    case VOCE_ERR_SCRATCH: return "RVE workspace larger than the static no-allocation workspace (96 MiB)";
     }
     return "unknown error";
 }
@@ -317,6 +343,7 @@ extern "C" void voce_ctx_destroy(voce_ctx *c)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->h_ms) (void)hipHostFree(c->h_ms);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_tab) (void)hipHostFree(c->h_tab);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -734,8 +761,13 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         d.part = (double *)(arena + o_PT);
     }
     if (opt) c->on_tab = opt->on_tab;
-    if (hipHostMalloc((void **)&c->h_ms, sizeof(MacroState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_ctl, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    c->log_e.reserve(c->log_cap);
+    c->log_s.reserve(c->log_cap);
+    c->log_v.reserve(c->log_cap);
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_ms, sizeof(MacroState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_ctl, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_tab, sizeof(RveDev), hipHostMallocDefault) != hipSuccess) {
         free(ms0);
         voce_ctx_destroy(c);
         return VOCE_ERR_NOMEM;
@@ -816,6 +848,15 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
 {
     if (!c) return VOCE_ERR_ARG;
     if (!c->d.spec) return VOCE_ERR_ARG;   // a server's resident RVE: increments go through the server
+    return voce_run_increment(c, rep, c->prof);
+}
+
+// One increment over the descriptor the context's device table holds (its
+// own workspace, or one a caller has put there).  timing: sample kernel
+// times every timing-th iteration (0: none; no events are created then, and
+// with the log vectors reserved at creation nothing is allocated).
+int voce_run_increment(voce_ctx *c, voce_incr_report *rep, int timing)
+{
     RveDev &d = c->d;
     int e = voce_launch_begin_increment(&c->L, c->stream);
     if (e) return VOCE_ERR_DEVICE;
@@ -839,7 +880,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
     while (launched < imax && c->h_ctl[3]) {
         int n = chunk;
         if (launched + n > imax) n = imax - launched;
-        if (c->prof && (int)c->ev.size() < 6 * n) {
+        if (timing && (int)c->ev.size() < 6 * n) {
             const size_t old = c->ev.size();
             c->ev.resize(6 * n);
             for (size_t i = old; i < c->ev.size(); ++i) CK(hipEventCreate(&c->ev[i]));
@@ -847,7 +888,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
         // timed iterations: every prof-th FFT iteration, the residue moving
         // by one per increment (early iterations of an increment run more
         // Newton steps; over prof increments every iteration index is sampled)
-        auto timed = [&](int it) { return c->prof > 0 && (it + c->n_steps_done) % c->prof == 0; };
+        auto timed = [&](int it) { return timing > 0 && (it + c->n_steps_done) % timing == 0; };
         for (int i = 0; i < n; ++i) {
             e = voce_launch_iteration(&c->L, c->stream, timed(launched + i) ? (void *)&c->ev[6 * i] : nullptr);
             if (e) return VOCE_ERR_DEVICE;
@@ -858,7 +899,7 @@ extern "C" int voce_increment(voce_ctx *c, voce_incr_report *rep)
         CK(hipMemcpyAsync(c->h_ctl, (char *)d.ms + ctl_off, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         CK(hipStreamSynchronize(c->stream));
         if (c->h_ctl[4]) return c->h_ctl[4];
-        if (c->prof) {
+        if (timing) {
             // only the iterations that ran (the rest returned at the flag check)
             const int ran = c->h_ctl[1] - done_before;
             for (int i = 0; i < ran && i < n; ++i) {

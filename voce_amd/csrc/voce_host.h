@@ -26,6 +26,8 @@ struct voce_ctx {
     LaunchSet L;           // launch sequence over that table
     MacroState *h_ms;      // pinned mirror of the device MacroState
     int *h_ctl;            // pinned: istep, iter, iter_total, active, error
+    RveDev *h_tab;         // pinned staging for a descriptor written to dtab
+    int device;            // HIP device the context was created on
     int two_crss_any;
     int log_cap;
     double *logs_dev;      // 3 x log_cap: Err_e, Err_s, s_vm per iteration
@@ -53,4 +55,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
 int voce_dev_alloc(voce_ctx *c, void **p, size_t bytes);
 int voce_sync_tab(voce_ctx *c);
 int voce_read_logs(voce_ctx *c, int n);
+int voce_run_increment(voce_ctx *c, voce_incr_report *rep, int timing);
+size_t voce_ws_bytes(const RveDev &d);       // per-increment workspace bytes of one RVE
+void voce_ws_carve(RveDev &d, char *p);      // point d's workspace fields into p
 void voce_fill_report(voce_ctx *c, const MacroState &M, voce_incr_report *rep);

@@ -108,27 +108,6 @@ struct Ws {
     bool held = false;
 };
 
-size_t ws_bytes(const RveDev &d, bool f32)
-{
-    const size_t cb = f32 ? 8 : 16, db = sizeof(double);
-    return al256(6 * d.V * db) + al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db) + al256(6 * d.nspec * cb) +
-           al256(3 * d.nspec * cb) + al256((size_t)d.nblk_vox * 16 * db);
-}
-
-void ws_carve(RveDev &d, char *p)
-{
-    const size_t cb = d.fft_f32 ? 8 : 16, db = sizeof(double);
-    d.P6 = (double *)p;
-    p += al256(6 * d.V * db);
-    d.KCV = (double *)p;
-    p += al256(VOCE_KCV_COMP * voce_kcv_voxels(d.V) * db);
-    d.spec = p;
-    p += al256(6 * d.nspec * cb);
-    d.usum = p;
-    p += al256(3 * d.nspec * cb);
-    d.part = (double *)p;
-}
-
 // FP64 copy of a row's evolving state, for an audit
 struct Shadow {
     int row = -1;            // table row
@@ -276,13 +255,13 @@ static int upload_row(voce_server *s, int row)
 static bool take_ws(voce_server *s, int row, RveDev &d)
 {
     Ws &w = s->ws[row];
-    const size_t n = ws_bytes(d, d.fft_f32 != 0);
+    const size_t n = voce_ws_bytes(d);
     size_t off = 0;
     if (!s->pool.alloc(n, &off)) return false;
     w.off = off;
     w.len = n;
     w.held = true;
-    ws_carve(d, s->pool.base + off);
+    voce_ws_carve(d, s->pool.base + off);
     return true;
 }
 
@@ -371,7 +350,7 @@ extern "C" int voce_server_submit(voce_server *s, int id, const double L[9], int
 {
     if (!s || !L || id < 0 || id >= s->o.max_rves || !s->ctx[id]) return VOCE_ERR_ARG;
     // a workspace larger than the whole pool would never be admitted
-    if (ws_bytes(s->ctx[id]->d, s->ctx[id]->d.fft_f32 != 0) > s->pool.cap) return VOCE_ERR_NOMEM;
+    if (voce_ws_bytes(s->ctx[id]->d) > s->pool.cap) return VOCE_ERR_NOMEM;
     Request r;
     r.tag = tag;
     r.id = id;
