@@ -90,9 +90,12 @@ def run_gpu(path, base, precision, steps, warmup, prof=True, tune=True):
         # costs queue time, a sparse sample keeps the timed run honest
         s.prof_enable(True, every=PROF_EVERY)
     reps = []
+    t_inc = []
     t0 = time.perf_counter()
     for k in range(steps):
+        t1 = time.perf_counter()
         reps.append(s.increment())
+        t_inc.append(time.perf_counter() - t1)
         if (k + 1) % 20 == 0:
             log(precision, "increment", k + 1, "iters", reps[-1]["iter"], "elapsed %.2f s" % (time.perf_counter() - t0))
     wall = time.perf_counter() - t0
@@ -100,6 +103,7 @@ def run_gpu(path, base, precision, steps, warmup, prof=True, tune=True):
     V = s.nvox
     s.close()
     run_gpu.tuned = tuned
+    run_gpu.t_inc = t_inc
     return reps, wall, kp, V
 
 
@@ -157,16 +161,18 @@ def run_batched(d, precision, n_rve, steps, log_every=5):
             "graph_replays": info["graph_replays"], "device_bytes": info["device_bytes"]}, reps_all
 
 
-def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5):
+def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5, precision="mixed"):
     """Config-5 style stream: n_ids resident RVEs, Poisson arrivals of
     one-increment requests (random traceless L, |L| = 1e-3/s, dt 1 s),
-    mixed precision with an FP64 audit of audit_pct % of the requests."""
+    served in `precision` with an FP64 audit of audit_pct % of the requests.
+    The first stream (all requests at t = 0) measures capacity; the same
+    seed gives the same ids and requests in either precision."""
     import voce_amd
     from voce_amd import inputs
     specs = inputs.serve_specs(n_ids=n_ids)
     base = inputs.full_velocity_case(time_step=1.0, err=1e-5, iter_max=100)
     t0 = time.perf_counter()
-    srv = voce_amd.Server(n_ids, "mixed", tune=True)
+    srv = voce_amd.Server(n_ids, precision, tune=True)
     inputs.write_case(os.path.join(d, "serve"), base, grain=np.ones(base.nvox, dtype=np.int32),
                       euler=np.zeros((1, 3)))
     path = os.path.join(d, "serve", "input")
@@ -181,7 +187,8 @@ def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5):
             log("served: registered %d ids (%.1f s)" % (i + 1, time.perf_counter() - t0))
     t_setup = time.perf_counter() - t0
     info0 = srv.info()
-    log("served: %d ids resident, %.1f GB, setup %.1f s" % (n_ids, info0["resident_bytes"] / 1e9, t_setup))
+    log("served %s: %d ids resident, %.1f GB, setup %.1f s" % (precision, n_ids, info0["resident_bytes"] / 1e9,
+                                                                t_setup))
     rng = inputs.SplitMix64(seed)
 
     def stream(rate, n, tag0):
@@ -227,7 +234,7 @@ def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5):
                 "audits": len(aud), "audit_sig_rel_max": (max(aud) if aud else None)}
 
     cap = stream(None, n_req, 0)
-    log("served capacity: %.1f increments/s, p50 %.1f ms p99 %.1f ms" % (cap["increments_per_s"],
+    log("served %s capacity: %.1f increments/s, p50 %.1f ms p99 %.1f ms" % (precision, cap["increments_per_s"],
                                                                         cap["latency_ms_p50"], cap["latency_ms_p99"]))
     sweep = []
     for j, f in enumerate(rates):
@@ -239,8 +246,8 @@ def run_served(d, n_ids, n_req, rates, audit_pct=1.0, seed=5):
                                                                                 r["latency_ms_p99"]))
     info = srv.info()
     srv.close()
-    return {"ids": n_ids, "resident_gb": round(info0["resident_bytes"] / 1e9, 2), "setup_s": round(t_setup, 1),
-            "capacity": cap, "sweep": sweep, "steps": info["steps"]}
+    return {"ids": n_ids, "precision": precision, "resident_gb": round(info0["resident_bytes"] / 1e9, 2),
+            "setup_s": round(t_setup, 1), "capacity": cap, "sweep": sweep, "steps": info["steps"]}
 
 
 def main():
@@ -254,9 +261,10 @@ def main():
     ap.add_argument("--batch-rves", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve-ids", type=int, default=1024)
+    ap.add_argument("--serve-ids", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-serve-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the RVE lives whole on one MI355X and nothing here "
@@ -327,11 +335,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "single RVE, 64^3 voxels, 125 Voronoi grains (seed 2), FCC Cu-like, x-tension "
-                               "1e-3/s, 100 increments dt 0.5, Err 1e-5, IterMax 100",
+                               "1e-3/s, dt 0.5, Err 1e-5, IterMax 100 (the 100-increment run); timed: increments "
+                               "1-%d of a fresh context after %d warm-up increments on another" % (args.steps,
+                                                                                                   args.warmup),
                    "voxels": V, "increments": args.steps, "precision": "fp64"},
         "rve_increments_per_s": round(args.steps / wall, 3),
         "fft_iterations": iters,
-        "increment_ms_p50": None,
+        "increment_ms": {"p50": round(float(np.percentile(run_gpu.t_inc, 50)) * 1e3, 3),
+                         "p99": round(float(np.percentile(run_gpu.t_inc, 99)) * 1e3, 3),
+                         "what": "one increment of the single 64^3 FP64 RVE, call to return"},
         "roofline": roof,
         "roofline_fp64": roof_fp64,
         "roofline_path": roof_path,
@@ -342,10 +354,10 @@ def main():
     if not args.no_mixed:
         mreps, mwall, _, _ = run_gpu(path, d, "mixed", args.steps, 0, prof=False)
         miters = sum(r["iter"] for r in mreps)
-        import numpy as np
         err = max(float(np.abs(a["sig_avg"] - b["sig_avg"]).max() / np.abs(b["sig_avg"]).max())
                   for a, b in zip(mreps, reps))
         out["mixed"] = {"value": round(V * miters / mwall, 1), "unit": "voxel-iterations/s",
+                        "over_fp64": round((V * miters / mwall) / vox_it, 3),
                         "rve_increments_per_s": round(args.steps / mwall, 3), "fft_iterations": miters,
                         "ms_per_step": round(mwall / args.steps * 1e3, 3),
                         "macro_stress_max_rel_err_vs_fp64": err}
@@ -364,15 +376,28 @@ def main():
                                       "1e-3/s, dt 1 s, %d increments, Err 1e-5, IterMax 100" % (args.batch_rves,
                                                                                                   args.batch_steps),
                           "fp64": bf, "mixed": bm,
-                          "mixed_rve_inc_over_plain_fp64_single": round(bm["rve_increments_per_s"] / rve_inc_fp64, 2)}
+                          # same workload, precision alone
+                          "mixed_over_fp64_voxel_iterations": round(bm["voxel_iterations_per_s"] /
+                                                                    bf["voxel_iterations_per_s"], 3),
+                          # against the single 64^3 FP64 RVE: RVE-increments (a mix of smaller RVEs) and
+                          # voxel-iterations (work done)
+                          "mixed_rve_inc_over_plain_fp64_single": round(bm["rve_increments_per_s"] / rve_inc_fp64, 2),
+                          "mixed_voxel_it_over_plain_fp64_single": round(bm["voxel_iterations_per_s"] / vox_it, 3)}
     if not args.no_serve:
         sv = run_served(d, args.serve_ids, args.serve_requests, (0.25, 0.5, 1.0, 2.0))
         sv["workload"] = ("%d resident ids, N^3 with N in {16,32,64}, 8-64 grains; Poisson arrivals of one-increment "
                           "requests, random traceless L (|L| = 1e-3/s), dt 1 s; mixed precision, FP64 audit of 1%%"
                           % args.serve_ids)
         sv["capacity_over_plain_fp64_single"] = round(sv["capacity"]["increments_per_s"] / rve_inc_fp64, 2)
+        sv["capacity_voxel_it_over_plain_fp64_single"] = round(sv["capacity"]["voxel_iterations_per_s"] / vox_it, 3)
+        if not args.no_serve_fp64:
+            # the same ids and the same capacity stream served in FP64
+            sf = run_served(d, args.serve_ids, args.serve_requests, (), audit_pct=0.0, precision="fp64")
+            sv["fp64_capacity"] = sf["capacity"]
+            sv["fp64_setup_s"] = sf["setup_s"]
+            sv["mixed_over_fp64_capacity"] = round(sv["capacity"]["increments_per_s"] /
+                                                   sf["capacity"]["increments_per_s"], 3)
         out["served"] = sv
-        out["increment_ms_p50"] = sv["sweep"][1]["latency_ms_p50"] if len(sv["sweep"]) > 1 else None
 
     if not args.no_cpu:
         try:
