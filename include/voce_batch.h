@@ -21,7 +21,8 @@ extern "C" {
 
 /* Launch choices of the three FFT kernels for one grid size and precision:
  * plane-kernel variant (0: 64 threads / planes <= 32x32, 1: 256 / <= 64x64,
- * 2: 512 / <= 128x128, 3: 320 / <= 64x64), x-pass variant (0: 256 threads,
+ * 2: 512 / <= 128x128, 3: 320 / <= 64x64, 4: line-parallel, square planes
+ * of 16, 32 or 64 only), x-pass variant (0: 256 threads,
  * 1: 128, 2: 64),
  * x-pass tile (columns per block), and whether radix-8 stages are used in
  * the plane and x passes.  us_* are the measured launch times (microseconds)

@@ -23,9 +23,13 @@ def write(tmp, case, grain=None, euler=None):
     return os.path.join(d, "input"), d
 
 
-def gpu_run(path, base, n, precision="fp64", diag=True, fields=()):
+def gpu_run(path, base, n, precision="fp64", diag=True, fields=(), tuning=None):
     cs = voce_amd.load_case(path, base)
     s = voce_amd.Solver.from_case(cs, precision)
+    if tuning:
+        t = s.tuning()
+        t.update(tuning)
+        s.apply_tuning(t)
     if diag:
         s.set_diagnostics(True)
     reps = [s.increment() for _ in range(n)]

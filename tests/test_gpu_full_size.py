@@ -19,11 +19,13 @@ from test_gpu_parity import compare
 pytestmark = pytest.mark.gpu
 
 
-def test_config2_64cube_against_oracle(tmp_path):
+@pytest.mark.parametrize("plv", [None, 4])
+def test_config2_64cube_against_oracle(tmp_path, plv):
     c = inputs.config_case(2)
     c.iter_max = 10
     c.n_steps = 2
-    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "svm", "evm"))
+    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "svm", "evm"),
+            tuning={"plv": plv} if plv else None)
 
 
 def _config3(tmp_path):

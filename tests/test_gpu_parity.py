@@ -19,10 +19,10 @@ FIELD_TOL = {"sig": 1e-8, "eps": 1e-8, "edot": 1e-7, "crss": 1e-10, "tranmat": 1
              "work_p": 1e-7, "rho_rss": 1e-8, "rho_crss": 1e-10, "work_e": 1e-8, "new_position": 1e-8}
 
 
-def compare(tmp_path, case, n, grain=None, euler=None, fields=FIELDS, macro_tol=1e-8):
+def compare(tmp_path, case, n, grain=None, euler=None, fields=FIELDS, macro_tol=1e-8, tuning=None):
     path, base = write(tmp_path, case, grain, euler)
-    say("case", case.dims, case.n_grains, "seed", case.seed, "increments", n)
-    g, gf = gpu_run(path, base, n, fields=fields)
+    say("case", case.dims, case.n_grains, "seed", case.seed, "increments", n, "tuning", tuning)
+    g, gf = gpu_run(path, base, n, fields=fields, tuning=tuning)
     o, of = oracle_run(path, base, n, fields=fields)
     for k in range(n):
         say("incr", k + 1, "iters gpu/oracle", g[k]["iter"], o[k]["iter"],
@@ -188,3 +188,23 @@ def test_mixed_precision_within_1e4(tmp_path):
         assert r < 1e-4
     # per-voxel: FP32 FFT round-off, stated bound 1e-3 of the field maximum
     assert rel(mf["sig"], ff["sig"]) < 1e-3
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (12, 16, 16), (32, 32, 32)])
+def test_fp64_parity_line_parallel_planes(tmp_path, dims):
+    # plane variant 4: the line-parallel plane kernels of square power-of-two
+    # planes (any Nx), against the oracle
+    c = inputs.Case(dims=dims, n_grains=6, seed=19, n_steps=3)
+    compare(tmp_path, c, 3, tuning={"plv": 4})
+
+
+def test_mixed_line_parallel_planes_against_oracle(tmp_path):
+    c = inputs.Case(dims=(32, 32, 32), n_grains=8, seed=23, n_steps=3)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 3, "mixed", fields=("sig",), tuning={"plv": 4})
+    o, of = oracle_run(path, base, 3, fields=("sig",))
+    for k in range(3):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed (plane variant 4) vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-6

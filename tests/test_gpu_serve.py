@@ -304,3 +304,22 @@ def test_alone_equals_mix_config5_sizes(tmp_path):
         _same(A[t], B[t])
         if A[t]["audited"]:
             assert 0.0 < A[t]["audit_sig_rel"] < 1e-4 and A[t]["audit_sig_rel"] == B[t]["audit_sig_rel"]
+
+
+def test_fp64_server_with_audits(tmp_path):
+    """An FP64 server may audit too: the FP64 shadow runs the default launch
+    choices beside tuned rows of the same grid and precision (separate launch
+    sets), and agrees with the served result to round-off."""
+    cases = _cases(tmp_path)
+    reqs = _requests(10, seed=17)
+    s = voce_amd.Server(len(cases), "fp64", tune=True)
+    for i, cs in enumerate(cases):
+        s.add_rve(i, cs)
+    for rid, L, _, tag in reqs:
+        s.submit(rid, L, tag % 2 == 0, tag)
+    res = s.drain()
+    s.close()
+    assert len(res) == len(reqs)
+    aud = [r["audit_sig_rel"] for r in res if r["audited"]]
+    say("fp64 audits", aud)
+    assert len(aud) == 5 and max(aud) < 1e-10

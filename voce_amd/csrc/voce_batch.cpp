@@ -115,6 +115,7 @@ static int build_group_maps(voce_batch *b, Group &g)
     put(one, &L.map_one, &L.n_one);
     const RveDev &d0 = b->htab[g.members[0]];
     L.plv = d0.plv;
+    L.p2_n = d0.ny;
     L.xv = d0.xv;
     L.pl_lds = pl_lds;
     L.x_lds = x_lds;
@@ -223,8 +224,8 @@ static int tune_group_timed(voce_batch *b, Group &g)
     };
     voce_tune_choice best = keep, cand;
     double best_pl = 1e300, best_x = 1e300;
-    for (int plv = 0; plv < 4 && !rc; ++plv)
-        for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+    for (int plv = 0; plv < VOCE_N_PLV && !rc; ++plv)
+        for (int r8 = 1; r8 >= (plv == 4 ? 1 : 0) && !rc; --r8) {   // plv 4 has fixed radices
             cand = keep;
             cand.plv = plv;
             cand.pl_r8 = r8;

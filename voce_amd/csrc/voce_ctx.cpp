@@ -287,6 +287,7 @@ int voce_sync_tab(voce_ctx *c)
     L.n_vox = d.nblk_vox;
     L.n_one = 1;
     L.plv = d.plv;
+    L.p2_n = d.ny;
     L.xv = d.xv;
     L.pl_lds = voce_plane_lds(&d, d.plv);
     L.x_lds = voce_xgreen_lds(&d, d.xv, d.tk);
@@ -694,6 +695,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.two_crss = two_any;
     // default FFT launch choices (a tuned set replaces them)
     d.plv = (ny * d.nzh <= 2112) ? 1 : 2;
+    if (voce_plane_lds(&d, 4) > 0) d.plv = 4;   // line-parallel planes when the grid allows them
     d.xv = 0;
     d.tk = voce_xgreen_tile(nx, 0);
     d.pl_r8 = d.x_r8 = 1;

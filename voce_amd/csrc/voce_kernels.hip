@@ -488,6 +488,298 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     }
 }
 
+// ------------------------------------------------------------------
+// Line-parallel plane kernels for square power-of-two planes (Ny = Nz = N,
+// N = 16, 32, 64): every line of N points (a packed z-row pair, then a y
+// column of the half spectrum) belongs to a group of L lanes of one wave,
+// R = N / L points per lane.  A line is transformed by an R-point DFT in
+// registers, the twiddles W_N^(n1 k1), one exchange through the line's own
+// LDS slots and L-point DFTs in registers.  The exchange only involves the
+// lanes of one wave (whose LDS operations are performed in order), so a
+// plane pass has one block barrier, between its z and y halves, and no
+// index arithmetic beyond constant strides.  Used when the tuner picks
+// plane variant 4 for a grid.
+// ------------------------------------------------------------------
+template <int N> struct P2Plan;
+template <> struct P2Plan<16> { static constexpr int L = 4, R = 4, NT = 64; };
+template <> struct P2Plan<32> { static constexpr int L = 4, R = 8, NT = 128; };
+template <> struct P2Plan<64> { static constexpr int L = 8, R = 8, NT = 320; };
+
+// Orders the LDS accesses of one wave (no instruction is emitted: the
+// accesses of a wave are performed in program order; this keeps the
+// compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// N-point DFT (forward, or inverse with INV) of the line held by the L lanes
+// of a group: on entry the lane with index n1 in its group holds x[n1 + L r]
+// in v[r]; on exit it holds X[n1 + L j + R k2] in v[j L + k2] (j < R / L,
+// k2 < L).  line(i) is the address of the i-th of the line's N LDS slots
+// (owned by the group; every lane must have read what it needs of them
+// before the call).  tw[k1] = W_N^(n1 k1), forward sign.
+//   X[k1 + R k2] = sum_n1 W_L^(n1 k2) W_N^(n1 k1) sum_r x[n1 + L r] W_R^(r k1)
+template <typename C, int N, bool INV, typename LINE>
+__device__ __forceinline__ void line_dft(C (&v)[P2Plan<N>::R], const C (&tw)[P2Plan<N>::R], int n1, LINE line)
+{
+    constexpr int L = P2Plan<N>::L, R = P2Plan<N>::R;
+    Bfly<R, INV, C>::run(v);
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) {
+        C w = tw[k1];
+        if (INV) w.y = -w.y;
+        v[k1] = cmul(v[k1], w);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) *line(k1 * L + n1) = v[k1];
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < R / L; ++j)
+#pragma unroll
+        for (int n = 0; n < L; ++n) v[j * L + n] = *line((n1 + L * j) * L + n);
+#pragma unroll
+    for (int j = 0; j < R / L; ++j) Bfly<L, INV, C>::run(&v[j * L]);
+}
+
+// index of the line element a lane holds in v[o] after line_dft
+template <int N> __device__ __forceinline__ constexpr int line_out_index(int n1, int o)
+{
+    return n1 + P2Plan<N>::L * (o / P2Plan<N>::L) + P2Plan<N>::R * (o % P2Plan<N>::L);
+}
+
+// Forward plane pass (the work of k_plane_fwd) for Ny = Nz = N.
+template <typename T, int N>
+__global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__restrict__ rtab,
+                                                                const int2 *__restrict__ bmap)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int L = P2Plan<N>::L, R = P2Plan<N>::R, NT = P2Plan<N>::NT;
+    constexpr int NZH = N / 2 + 1, RS = 2 * NZH, KT = N / (2 * L);
+    RVE_PROLOGUE(lb);
+    if (lb == 6 * r.nx) {   // the previous iteration's finaliser
+        finalize_block<NT>(r);
+        return;
+    }
+    if (!iter_active(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;   // the plane: row y at y NZH (rows 2p, 2p+1 = packed row p at p RS)
+    const int nx = r.nx;
+    const int c = lb / nx, x = lb - c * nx;
+    const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_z;   // tw_y holds the same table (Ny = Nz)
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+    }
+    // z: packed row pair p = grp, two real rows as one complex row
+    if (grp < N / 2) {
+        // the two rows in pairs of points, consecutive lanes on consecutive
+        // pairs (whole cache lines per instruction), into the line's slots in
+        // natural order, then read back in the layout line_dft takes
+        const double2 *src = (const double2 *)(r.S6 + (long)c * r.V + ((long)x * N + 2 * grp) * N);
+        C *line = s + grp * RS;
+        double2 pa[R / 2], pb[R / 2];
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            pa[q] = src[n1 + L * q];
+            pb[q] = src[N / 2 + n1 + L * q];
+        }
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            const int z = 2 * (n1 + L * q);
+            line[z] = C{(T)pa[q].x, (T)pb[q].x};
+            line[z + 1] = C{(T)pa[q].y, (T)pb[q].y};
+        }
+        wave_lds_sync();
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = line[n1 + L * q];
+        line_dft<C, N, false>(v, tw, n1, [&](int i) { return line + i; });
+        // split Z = a + i b into the half spectra of a and b:
+        // A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / 2i, k <= N/2
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) line[line_out_index<N>(n1, o)] = v[o];
+        wave_lds_sync();
+        C A[KT + 1], B[KT + 1];
+#pragma unroll
+        for (int t = 0; t <= KT; ++t) {
+            const int k = n1 + L * t;
+            if (k <= N / 2) {
+                const C z0 = line[k], z1 = line[(N - k) & (N - 1)];
+                A[t] = C{(T)0.5 * (z0.x + z1.x), (T)0.5 * (z0.y - z1.y)};
+                B[t] = C{(T)0.5 * (z0.y + z1.y), (T)0.5 * (z1.x - z0.x)};
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t <= KT; ++t) {
+            const int k = n1 + L * t;
+            if (k <= N / 2) {
+                line[k] = A[t];
+                line[NZH + k] = B[t];
+            }
+        }
+    }
+    __syncthreads();
+    // y: half-spectrum column kz = grp
+    if (grp < NZH) {
+        C *col = s + grp;
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = col[(n1 + L * q) * NZH];
+        line_dft<C, N, false>(v, tw, n1, [&](int i) { return col + i * NZH; });
+        constexpr int TB = 128 / (int)sizeof(C);
+        C *dst = (C *)r.spec + (long)c * r.nspec;
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            const int e = line_out_index<N>(n1, o) * NZH + grp;
+            dst[((long)(e / TB) * nx + x) * TB + (e % TB)] = v[o];
+        }
+    }
+}
+
+// Inverse plane pass (the work of k_plane_inv, both modes) for Ny = Nz = N.
+template <typename T, int N>
+__global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__restrict__ rtab,
+                                                                const int2 *__restrict__ bmap, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int L = P2Plan<N>::L, R = P2Plan<N>::R;
+    constexpr int NZH = N / 2 + 1, RS = 2 * NZH;
+    RVE_PROLOGUE(lb);
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx;
+    const int c = lb / nx, x = lb - c * nx;
+    const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
+    const MacroState *ms = r.ms;
+    const bool creep_set = mode == 0 && (ms->creep != 0) && (ms->iter == 0);
+    const long off = (long)c * r.V + ((long)x * N + 2 * grp) * N;   // row 2p of this group
+    // E of the two rows this lane updates, loaded now so that the loads
+    // are in flight through both halves of the pass
+    // (pairs of points, consecutive lanes on consecutive pairs)
+    double2 ev[2][R / 2];
+    if (mode == 0 && !creep_set && grp < N / 2) {
+        const double2 *E = (const double2 *)(r.E6 + off);
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            ev[0][q] = E[n1 + L * q];
+            ev[1][q] = E[N / 2 + n1 + L * q];
+        }
+    }
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_z;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+    }
+    // y: half-spectrum column kz = grp, inverse DFT along y
+    if (grp < NZH) {
+        constexpr int TB = 128 / (int)sizeof(C);
+        const C *srcp = (const C *)r.spec + (long)c * r.nspec;
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int e = (n1 + L * q) * NZH + grp;
+            v[q] = srcp[((long)(e / TB) * nx + x) * TB + (e % TB)];
+        }
+        C *col = s + grp;
+        line_dft<C, N, true>(v, tw, n1, [&](int i) { return col + i * NZH; });
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) col[line_out_index<N>(n1, o) * NZH] = v[o];
+    }
+    __syncthreads();
+    // z: rows 2p (a) and 2p+1 (b) as one complex row Z = A + iB, A and B
+    // Hermitian-extended from their half spectra, inverse DFT along z
+    if (grp < N / 2) {
+        C *line = s + grp * RS;
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int k = n1 + L * q;
+            C a, b;
+            if (k <= N / 2) {
+                a = line[k];
+                b = line[NZH + k];
+                if (k == 0 || 2 * k == N) { a.y = (T)0; b.y = (T)0; }
+            } else {
+                a = line[N - k];
+                b = line[NZH + N - k];
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            v[q] = C{a.x - b.y, a.y + b.x};
+        }
+        line_dft<C, N, true>(v, tw, n1, [&](int i) { return line + i; });
+        // back through the line's slots to pairs of consecutive points
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) line[line_out_index<N>(n1, o)] = v[o];
+        wave_lds_sync();
+        double2 va[R / 2], vb[R / 2];
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            const int z = 2 * (n1 + L * q);
+            const C u0 = line[z], u1 = line[z + 1];
+            va[q] = double2{(double)u0.x, (double)u1.x};
+            vb[q] = double2{(double)u0.y, (double)u1.y};
+        }
+        if (mode == 0) {
+            double2 *E = (double2 *)(r.E6 + off);
+            const double sh = creep_set ? 0.0 : ms->shift6[c];
+            const double cr = ms->creep_sym6[c];
+#pragma unroll
+            for (int q = 0; q < R / 2; ++q) {
+                const int i = n1 + L * q;
+                if (creep_set) {
+                    E[i] = double2{cr + (sh + va[q].x), cr + (sh + va[q].y)};
+                    E[N / 2 + i] = double2{cr + (sh + vb[q].x), cr + (sh + vb[q].y)};
+                } else {
+                    E[i] = double2{ev[0][q].x + (sh + va[q].x), ev[0][q].y + (sh + va[q].y)};
+                    E[N / 2 + i] = double2{ev[1][q].x + (sh + vb[q].x), ev[1][q].y + (sh + vb[q].y)};
+                }
+            }
+            // diagnostics: the running sum of the Green updates (the
+            // reference's fluctuation, evolution.c:392) for the components
+            // new_position reads
+            const int fs = (c == 1) ? 0 : (c == 2) ? 1 : (c == 3) ? 2 : (c == 5) ? 3 : -1;
+            if (r.FLU && fs >= 0) {
+                double2 *F = (double2 *)(r.FLU + (long)fs * r.V + ((long)x * N + 2 * grp) * N);
+#pragma unroll
+                for (int q = 0; q < R / 2; ++q) {
+                    const int i = n1 + L * q;
+                    const double2 fa = F[i], fb = F[N / 2 + i];
+                    F[i] = double2{fa.x + va[q].x, fa.y + va[q].y};
+                    F[N / 2 + i] = double2{fb.x + vb[q].x, fb.y + vb[q].y};
+                }
+            }
+        } else {
+            double2 *W = (double2 *)(r.WF + off);
+            double2 *F = (r.FLU && c < 2) ? (double2 *)(r.FLU + (long)(4 + c) * r.V + ((long)x * N + 2 * grp) * N)
+                                          : nullptr;
+#pragma unroll
+            for (int q = 0; q < R / 2; ++q) {
+                const int i = n1 + L * q;
+                W[i] = va[q];
+                W[N / 2 + i] = vb[q];
+                if (F) {
+                    const double2 fa = F[i], fb = F[N / 2 + i];
+                    F[i] = double2{fa.x + va[q].x, fa.y + va[q].y};
+                    F[N / 2 + i] = double2{fb.x + vb[q].x, fb.y + vb[q].y};
+                }
+            }
+        }
+    }
+}
+
 // unit frequency vector, init.c:660-677
 template <typename T>
 __device__ __forceinline__ void kvec(int kx, int ky, int kz, int nx, int ny, int nz, T n[3])
@@ -974,44 +1266,43 @@ __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *o
 // Fixed-order sum of nb partial vectors (NV values each, stride 16) by one
 // block; the result lands in tot[] (shared).  Reads bypass L1 so that the
 // partials other blocks wrote in this launch are seen.
+// Sums rows 0..nb-1 of 16-double partial rows (the first NV entries) into
+// tot in an order that does not depend on the block size: 16 row streams,
+// stream s summing rows s, s + 16, s + 32, ... in row order, then the streams
+// added in stream order.  Lane l of a wave takes entry l % 16 of stream
+// 4 g + l / 16 of stream group g, so a load instruction reads four whole
+// 128-byte rows; four waves take one group each (fewer waves loop over the
+// groups).  Ends with a barrier.
 template <int NT, int NV>
 __device__ __forceinline__ void sum_partials(const double *part, int nb, double *tot)
 {
-    __shared__ double red[NT / 64][NV];
-    double a[NV];
+    constexpr int NW = NT / 64, NWU = NW < 4 ? NW : 4, PASSES = 4 / NWU, U = 16;
+    static_assert(NW == 1 || NW == 2 || NW >= 4, "waves per block");
+    __shared__ double red[16][16];
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int e = lane & 15;
+    if (w < NWU) {
+#pragma unroll 1
+        for (int t = 0; t < PASSES; ++t) {
+            const int st = 4 * (w + NWU * t) + (lane >> 4);
+            double a = 0.0;
+            int b = st;
+            for (; b + 16 * (U - 1) < nb; b += 16 * U) {
+                double q[U];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) a[i] = 0.0;
-    // rows b, b + NT, b + 2 NT, ... in that order; four rows' loads in flight at once
-    int b = threadIdx.x;
-    for (; b + 3 * NT < nb; b += 4 * NT) {
-        double q[4][NV];
+                for (int k = 0; k < U; ++k) q[k] = __builtin_nontemporal_load(part + (long)(b + 16 * k) * 16 + e);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) q[k][i] = __builtin_nontemporal_load(part + (long)(b + k * NT) * 16 + i);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) a[i] += q[k][i];
+                for (int k = 0; k < U; ++k) a += q[k];
+            }
+            for (; b < nb; b += 16) a += __builtin_nontemporal_load(part + (long)b * 16 + e);
+            red[st][e] = a;
+        }
     }
-    for (; b < nb; b += NT)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) a[i] += __builtin_nontemporal_load(part + (long)b * 16 + i);
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double x = a[i];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        a[i] = x;
-    }
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) red[threadIdx.x >> 6][i] = a[i];
     __syncthreads();
     if (threadIdx.x < NV) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
+        for (int k = 0; k < 16; ++k) acc += red[k][threadIdx.x];
         tot[threadIdx.x] = acc;
     }
     __syncthreads();
@@ -2067,6 +2358,13 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
     case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_pfwd), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_pfwd); break;
     case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_pfwd), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_pfwd); break;
     case 3: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>), dim3(L->n_pfwd), dim3(NT_PL_M), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 4: {   // line-parallel, square power-of-two planes (the grid's ny)
+        const int n = L->p2_n;
+        if (n == 16) hipLaunchKernelGGL((k_plane_fwd_p2<T, 16>), dim3(L->n_pfwd), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else if (n == 32) hipLaunchKernelGGL((k_plane_fwd_p2<T, 32>), dim3(L->n_pfwd), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else hipLaunchKernelGGL((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        break;
+    }
     default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
@@ -2080,6 +2378,13 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
     case 0: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_T, MAXEL_PL_T>), dim3(n), dim3(NT_PL_T), L->pl_lds, st, L->tab, map, mode); break;
     case 1: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(n), dim3(NT_PL_S), L->pl_lds, st, L->tab, map, mode); break;
     case 3: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_M, MAXEL_PL_M>), dim3(n), dim3(NT_PL_M), L->pl_lds, st, L->tab, map, mode); break;
+    case 4: {
+        const int m = L->p2_n;
+        if (m == 16) hipLaunchKernelGGL((k_plane_inv_p2<T, 16>), dim3(n), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, map, mode);
+        else if (m == 32) hipLaunchKernelGGL((k_plane_inv_p2<T, 32>), dim3(n), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, map, mode);
+        else hipLaunchKernelGGL((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
+        break;
+    }
     default: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
     }
 }
@@ -2143,7 +2448,10 @@ hipError_t set_lds_limits()
                          (const void *)k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>, (const void *)k_plane_inv<T, NT_PL_L, MAXEL_PL_L>,
                          (const void *)k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>, (const void *)k_plane_inv<T, NT_PL_M, MAXEL_PL_M>,
                          (const void *)k_xgreen<T, NT_X, MAXEL_X>, (const void *)k_xgreen<T, NT_XB, MAXEL_XB>,
-                         (const void *)k_xgreen<T, NT_XC, MAXEL_XC>};
+                         (const void *)k_xgreen<T, NT_XC, MAXEL_XC>,
+                         (const void *)k_plane_fwd_p2<T, 16>, (const void *)k_plane_inv_p2<T, 16>,
+                         (const void *)k_plane_fwd_p2<T, 32>, (const void *)k_plane_inv_p2<T, 32>,
+                         (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>};
     for (const void *f : fns) {
         // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
         hipFuncAttributes fa;
@@ -2199,8 +2507,15 @@ static const size_t PLANE_STATIC_LDS = sizeof(MacroState) + 16 * 8 + (NT_PL_L / 
 extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 {
     const int el = r->ny * r->nzh;
-    if (plv < 0 || plv > 3 || el > PL_MAXEL[plv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
+    if (plv == 4) {
+        // line-parallel variant: square planes of 16, 32 or 64, the plane only
+        const int n = r->ny;
+        if (r->nz != n || (n != 16 && n != 32 && n != 64)) return 0;
+        const size_t b = (size_t)el * cb;
+        return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
+    }
+    if (plv < 0 || plv > 3 || el > PL_MAXEL[plv]) return 0;
     const size_t b = (size_t)(el + r->ny + r->nz) * cb;
     return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
 }

@@ -467,6 +467,7 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     d.tw_z = p;
     // FP64 launch choices (defaults; audits are not tuned)
     d.plv = (d.ny * d.nzh <= 2112) ? 1 : 2;
+    if (voce_plane_lds(&d, 4) > 0) d.plv = 4;
     d.xv = 0;
     d.tk = voce_xgreen_tile(d.nx, 0);
     d.pl_r8 = d.x_r8 = 1;
@@ -511,10 +512,12 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
 static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<LaunchSet> &sets, int2 *h, int2 *dmaps,
                       size_t cap, size_t *used)
 {
-    std::map<std::tuple<int, int, int, int>, std::vector<int>> g;
+    // one launch set per grid, precision and kernel variants (an audit's
+    // FP64 shadow may run other variants than a tuned row of the same grid)
+    std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> g;
     for (int r : rows) {
         const RveDev &d = s->htab[r];
-        g[std::make_tuple(d.nx, d.ny, d.nz, d.fft_f32)].push_back(r);
+        g[std::make_tuple(d.nx, d.ny, d.nz, d.fft_f32, d.plv, d.xv)].push_back(r);
     }
     sets.clear();
     size_t off = 0;
@@ -524,6 +527,7 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
         L.tab = s->dtab;
         const RveDev &d0 = s->htab[kv.second[0]];
         L.plv = d0.plv;
+        L.p2_n = d0.ny;
         L.xv = d0.xv;
         L.fft_f32 = d0.fft_f32;
         L.uniform_phase = 1;

@@ -121,8 +121,8 @@ int measure(voce_ctx *c, voce_tune_choice *best)
     double best_pl = 1e300, best_x = 1e300;
     voce_tune_choice cand = keep, b = keep;
     // plane kernels: variant x radix set (x-pass choice held at its current value)
-    for (int plv = 0; plv < 4 && !rc; ++plv)
-        for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+    for (int plv = 0; plv < VOCE_N_PLV && !rc; ++plv)
+        for (int r8 = 1; r8 >= (plv == 4 ? 1 : 0) && !rc; --r8) {   // plv 4 has fixed radices
             cand = keep;
             cand.plv = plv;
             cand.pl_r8 = r8;
