@@ -19,13 +19,13 @@ from test_gpu_parity import compare
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("plv", [None, 4])
-def test_config2_64cube_against_oracle(tmp_path, plv):
+@pytest.mark.parametrize("tuning", [{"plv": 1, "xv": 0, "tk": 4}, {"plv": 4, "xv": 3, "tk": 2},
+                                    {"plv": 4, "xv": 3, "tk": 4}])
+def test_config2_64cube_against_oracle(tmp_path, tuning):
     c = inputs.config_case(2)
     c.iter_max = 10
     c.n_steps = 2
-    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "svm", "evm"),
-            tuning={"plv": plv} if plv else None)
+    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "svm", "evm"), tuning=tuning)
 
 
 def _config3(tmp_path):

@@ -190,18 +190,24 @@ def test_mixed_precision_within_1e4(tmp_path):
     assert rel(mf["sig"], ff["sig"]) < 1e-3
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (12, 16, 16), (32, 32, 32)])
-def test_fp64_parity_line_parallel_planes(tmp_path, dims):
-    # plane variant 4: the line-parallel plane kernels of square power-of-two
-    # planes (any Nx), against the oracle
+@pytest.mark.parametrize("dims,tuning", [((16, 16, 16), {"plv": 4}), ((12, 16, 16), {"plv": 4}),
+                                         ((32, 32, 32), {"plv": 4}),
+                                         ((16, 16, 16), {"plv": 4, "xv": 3, "tk": 4}),
+                                         ((16, 16, 16), {"plv": 4, "xv": 3, "tk": 8}),
+                                         ((32, 12, 16), {"xv": 3, "tk": 4}),
+                                         ((32, 32, 32), {"plv": 4, "xv": 3, "tk": 8})])
+def test_fp64_parity_line_parallel(tmp_path, dims, tuning):
+    # plane variant 4 (line-parallel kernels of square power-of-two planes,
+    # any Nx) and x-pass variant 3 (line-parallel x pass, Nx = 16, 32, 64, any
+    # plane), against the oracle
     c = inputs.Case(dims=dims, n_grains=6, seed=19, n_steps=3)
-    compare(tmp_path, c, 3, tuning={"plv": 4})
+    compare(tmp_path, c, 3, tuning=tuning)
 
 
 def test_mixed_line_parallel_planes_against_oracle(tmp_path):
     c = inputs.Case(dims=(32, 32, 32), n_grains=8, seed=23, n_steps=3)
     path, base = write(tmp_path, c)
-    g, gf = gpu_run(path, base, 3, "mixed", fields=("sig",), tuning={"plv": 4})
+    g, gf = gpu_run(path, base, 3, "mixed", fields=("sig",), tuning={"plv": 4, "xv": 3, "tk": 4})
     o, of = oracle_run(path, base, 3, fields=("sig",))
     for k in range(3):
         r = rel(g[k]["sig_avg"], o[k]["sig_avg"])

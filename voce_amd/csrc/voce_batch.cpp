@@ -116,6 +116,8 @@ static int build_group_maps(voce_batch *b, Group &g)
     const RveDev &d0 = b->htab[g.members[0]];
     L.plv = d0.plv;
     L.p2_n = d0.ny;
+    L.p2_nx = d0.nx;
+    L.p2_tk = d0.tk;
     L.xv = d0.xv;
     L.pl_lds = pl_lds;
     L.x_lds = x_lds;
@@ -240,9 +242,9 @@ static int tune_group_timed(voce_batch *b, Group &g)
             }
         }
     const int tks[5] = {1, 2, 4, 8, 16};
-    for (int xv = 0; xv < 3 && !rc; ++xv)
+    for (int xv = 0; xv < VOCE_N_XV && !rc; ++xv)
         for (int ti = 0; ti < 5 && !rc; ++ti)
-            for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+            for (int r8 = 1; r8 >= (xv == 3 ? 1 : 0) && !rc; --r8) {   // xv 3 has fixed radices
                 cand = keep;
                 cand.xv = xv;
                 cand.tk = tks[ti];

@@ -288,6 +288,8 @@ int voce_sync_tab(voce_ctx *c)
     L.n_one = 1;
     L.plv = d.plv;
     L.p2_n = d.ny;
+    L.p2_nx = d.nx;
+    L.p2_tk = d.tk;
     L.xv = d.xv;
     L.pl_lds = voce_plane_lds(&d, d.plv);
     L.x_lds = voce_xgreen_lds(&d, d.xv, d.tk);

@@ -123,7 +123,7 @@ struct RveDev {
     int uniform_phase;           // 1: one crystalline phase (its constants are uniform)
     int two_crss;                // 1: some phase keeps a separate backward CRSS
     // FFT launch choices (tuned per grid size): plane-kernel variant plv
-    // (0..4) and x-pass variant xv (0..2, see voce_kernels.hip), x-pass tile tk
+    // (0..4) and x-pass variant xv (0..3, see voce_kernels.hip), x-pass tile tk
     // ((y,kz) columns per block), radix-8 stages allowed in the plane and
     // x passes
     int plv, xv, tk, pl_r8, x_r8;
@@ -143,6 +143,7 @@ struct LaunchSet {
     int n_pfwd, n_plane6, n_plane3, n_x, n_vox, n_one;
     int plv, xv;                 // kernel variants (same for every RVE of the set)
     int p2_n;                    // plv 4: the square plane size ny = nz of the set
+    int p2_nx, p2_tk;            // xv 3: nx and the x-pass tile of the set
     size_t pl_lds, x_lds;        // dynamic LDS: the largest over the set's RVEs
     int fft_f32, uniform_phase, two_crss;
 };
@@ -167,6 +168,7 @@ int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_de
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 #define VOCE_N_PLV 5                // plane-kernel variants (4: line-parallel square power-of-two planes)
+#define VOCE_N_XV 4                 // x-pass variants (3: line-parallel, Nx 16, 32, 64)
 int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);
 int voce_vox_block(void);   // threads per block of the per-voxel kernels

@@ -1032,6 +1032,140 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
 }
 
 // ------------------------------------------------------------------
+// Line-parallel x pass for Nx = 16, 32, 64 (x-pass variant 3): one wave
+// owns all six component lines of CPW columns (CPW = 1 for Nx = 64, 2 below):
+// groups of L lanes run the forward x DFTs (line_dft), then every lane
+// applies the Green operator to one k point of the wave's columns (the six
+// components meet in the wave's LDS), then the groups run the inverse x DFTs.
+// No block barrier: the waves of a block (WPB of them, CPW * WPB columns,
+// the tile tk) are independent; the skew sums of the k points are loaded
+// at the start.  Mode 1 runs the inverse x DFT of the three skew sums.
+// ------------------------------------------------------------------
+template <int NX> struct XP2 {
+    static constexpr int L = P2Plan<NX>::L, CPW = (NX == 64) ? 1 : 2;
+};
+
+template <typename T, int NX, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap,
+                                                        int mode)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int L = XP2<NX>::L, R = P2Plan<NX>::R, CPW = XP2<NX>::CPW, CPB = CPW * WPB;
+    constexpr int TB = 128 / (int)sizeof(C);
+    RVE_PROLOGUE(lb);
+    const MacroState *ms = r.ms;
+    if (mode == 0 ? !iter_active(ms) : !rve_need_skew(ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ny = r.ny, nz = r.nz, nzh = r.nzh;
+    const int ncolt = ny * nzh;
+    // tiles sharing 128-byte lines on one XCD (as in k_xgreen)
+    int tile = lb;
+    {
+        const int G = TB / CPB, span = 8 * G;
+        const int nt = (ncolt + CPB - 1) / CPB;
+        if (G > 1 && lb < (nt / span) * span) {
+            const int xcd = lb & 7, j = lb >> 3;
+            tile = ((j / G) * 8 + xcd) * G + (j % G);
+        }
+    }
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int wc0 = tile * CPB + w * CPW;   // the wave's first column
+    C *ws = (C *)smem + w * (6 * CPW * NX);  // line (j, cc) of the wave at ws + (j * 6 + cc) * NX
+    const long cstr = (long)r.ncb * NX * TB;
+    auto gidx = [&](int col, int x) { return ((long)(col / TB) * NX + x) * TB + (col % TB); };
+    // DFT role: group g of L lanes holds component cc of column j
+    const int g = lane / L, n1 = lane % L;
+    const int cc = g % 6, j = g / 6, col = wc0 + j;
+    const bool dft = (g < 6 * CPW) && (col < ncolt) && (mode == 0 || cc < 3);
+    // Green role: lane p takes k point (column j2, kx)
+    const int j2 = lane / NX, kx = lane % NX, col2 = wc0 + j2;
+    const bool pt = (lane < CPW * NX) && (col2 < ncolt) && mode == 0;
+    const bool first = (ms->iter == 0);
+    C *usum = (C *)r.usum;
+    C u[3];
+    if (pt && !first) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u[i] = usum[i * cstr + gidx(col2, kx)];
+    }
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_x;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+    }
+    C *line = ws + (j * 6 + cc) * NX;
+    C *dst = (C *)r.spec;
+    if (mode == 1) {
+        // the skew sums back to x: inverse x DFT of USUM into SPEC[0..2]
+        if (dft) {
+            C v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = usum[cc * cstr + gidx(col, n1 + L * q)];
+            line_dft<C, NX, true>(v, tw, n1, [&](int i) { return line + i; });
+#pragma unroll
+            for (int o = 0; o < R; ++o) dst[cc * cstr + gidx(col, line_out_index<NX>(n1, o))] = v[o];
+        }
+        return;
+    }
+    if (dft) {
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = dst[cc * cstr + gidx(col, n1 + L * q)];
+        line_dft<C, NX, false>(v, tw, n1, [&](int i) { return line + i; });
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) line[line_out_index<NX>(n1, o)] = v[o];
+    }
+    wave_lds_sync();
+    if (pt) {
+        const int ky = col2 / nzh, kz = col2 - ky * nzh;
+        C *pl = ws + j2 * 6 * NX + kx;   // component c at pl[c * NX]
+        C a[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[c] = pl[c * NX];
+        C d[6], wk[3];
+        if (kx == 0 && ky == 0 && kz == 0) {
+            // Gamma(k = 0) = 0 (init.c:586-591)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) d[c] = C{0, 0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) wk[c] = C{0, 0};
+        } else {
+            const T invV = (T)(1.0 / (double)r.V);
+            T n[3];
+            kvec<T>(kx, ky, kz, NX, ny, nz, n);
+            green_apply<T, C>(green_coef<T>(ms), n, a, invV, d, wk);
+            // Hermitian average on the x and y Nyquist planes, as in k_xgreen
+            const bool nyq = (2 * kx == NX) || (2 * ky == ny);
+            if (nyq && kz > 0 && 2 * kz < nz) {
+                T m[3];
+                kvec<T>((NX - kx) % NX, (ny - ky) % ny, nz - kz, NX, ny, nz, m);
+                C d2[6], w2[3];
+                green_apply<T, C>(green_coef<T>(ms), m, a, invV, d2, w2);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) d[c] = C{(T)0.5 * (d[c].x + d2[c].x), (T)0.5 * (d[c].y + d2[c].y)};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) wk[c] = C{(T)0.5 * (wk[c].x + w2[c].x), (T)0.5 * (wk[c].y + w2[c].y)};
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) pl[c * NX] = d[c];
+        // sum of the skew parts over the increment (lattice spin)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) usum[i * cstr + gidx(col2, kx)] = first ? wk[i] : cadd(u[i], wk[i]);
+    }
+    wave_lds_sync();
+    if (dft) {
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = line[n1 + L * q];
+        line_dft<C, NX, true>(v, tw, n1, [&](int i) { return line + i; });
+#pragma unroll
+        for (int o = 0; o < R; ++o) dst[cc * cstr + gidx(col, line_out_index<NX>(n1, o))] = v[o];
+    }
+}
+
+// ------------------------------------------------------------------
 // Constitutive pieces (double, registers)
 // ------------------------------------------------------------------
 
@@ -2389,10 +2523,23 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
     }
 }
 
+template <typename T, int NX>
+void launch_xgreen_p2(const LaunchSet *L, hipStream_t st, int mode)
+{
+    const int wpb = L->p2_tk / XP2<NX>::CPW;
+    if (wpb == 4) hipLaunchKernelGGL((k_xgreen_p2<T, NX, 4>), dim3(L->n_x), dim3(256), L->x_lds, st, L->tab, L->map_x, mode);
+    else hipLaunchKernelGGL((k_xgreen_p2<T, NX, 2>), dim3(L->n_x), dim3(128), L->x_lds, st, L->tab, L->map_x, mode);
+}
+
 template <typename T>
 void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
 {
     switch (L->xv) {
+    case 3:   // line-parallel x pass (the grid's nx)
+        if (L->p2_nx == 16) launch_xgreen_p2<T, 16>(L, st, mode);
+        else if (L->p2_nx == 32) launch_xgreen_p2<T, 32>(L, st, mode);
+        else launch_xgreen_p2<T, 64>(L, st, mode);
+        break;
     case 0: hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
     case 1: hipLaunchKernelGGL((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
     default: hipLaunchKernelGGL((k_xgreen<T, NT_XC, MAXEL_XC>), dim3(L->n_x), dim3(NT_XC), L->x_lds, st, L->tab, L->map_x, mode); break;
@@ -2451,7 +2598,10 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen<T, NT_XC, MAXEL_XC>,
                          (const void *)k_plane_fwd_p2<T, 16>, (const void *)k_plane_inv_p2<T, 16>,
                          (const void *)k_plane_fwd_p2<T, 32>, (const void *)k_plane_inv_p2<T, 32>,
-                         (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>};
+                         (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>,
+                         (const void *)k_xgreen_p2<T, 16, 2>, (const void *)k_xgreen_p2<T, 16, 4>,
+                         (const void *)k_xgreen_p2<T, 32, 2>, (const void *)k_xgreen_p2<T, 32, 4>,
+                         (const void *)k_xgreen_p2<T, 64, 2>, (const void *)k_xgreen_p2<T, 64, 4>};
     for (const void *f : fns) {
         // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
         hipFuncAttributes fa;
@@ -2488,9 +2638,11 @@ static const int PL_MAXEL[4] = {MAXEL_PL_T, MAXEL_PL_S, MAXEL_PL_L, MAXEL_PL_M};
 static const int X_MAXEL[3] = {MAXEL_X, MAXEL_XB, MAXEL_XC};
 static const size_t LDS_MAX = 160 * 1024;
 
-// Largest x-pass tile (columns per block, <= 16) for variant xv.
+// Largest x-pass tile (columns per block, <= 16) for variant xv (variant 3:
+// its smaller tile, two waves).
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
+    if (xv == 3) return (nx == 16 || nx == 32) ? 4 : (nx == 64 ? 2 : 0);
     const int fit = X_MAXEL[xv] / (6 * nx);
     if (fit < 1) return 0;   // the variant cannot hold one column
     int tk = 1;
@@ -2522,8 +2674,16 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
 
 extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
 {
-    if (xv < 0 || xv > 2 || tk < 1 || (tk & (tk - 1)) || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
     const size_t cb = r->fft_f32 ? 8 : 16;
+    if (xv == 3) {
+        // line-parallel x pass: Nx 16, 32 or 64; tk = 2 or 4 waves of CPW
+        // columns; six lines of Nx per column
+        const int nx = r->nx, cpw = nx == 64 ? 1 : 2;
+        if (nx != 16 && nx != 32 && nx != 64) return 0;
+        if (tk != 2 * cpw && tk != 4 * cpw) return 0;
+        return (size_t)tk * 6 * nx * cb;
+    }
+    if (xv < 0 || xv > 2 || tk < 1 || (tk & (tk - 1)) || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
     // 6 spectra + 3 skew sums, lines padded to nx + 1, then the twiddles
     const size_t b = (size_t)(9 * tk * (r->nx + 1) + r->nx) * cb;
     return b <= LDS_MAX ? b : 0;

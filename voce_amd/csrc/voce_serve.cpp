@@ -514,10 +514,10 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
 {
     // one launch set per grid, precision and kernel variants (an audit's
     // FP64 shadow may run other variants than a tuned row of the same grid)
-    std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> g;
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<int>> g;
     for (int r : rows) {
         const RveDev &d = s->htab[r];
-        g[std::make_tuple(d.nx, d.ny, d.nz, d.fft_f32, d.plv, d.xv)].push_back(r);
+        g[std::make_tuple(d.nx, d.ny, d.nz, d.fft_f32, d.plv, d.xv, d.tk)].push_back(r);
     }
     sets.clear();
     size_t off = 0;
@@ -528,6 +528,8 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
         const RveDev &d0 = s->htab[kv.second[0]];
         L.plv = d0.plv;
         L.p2_n = d0.ny;
+        L.p2_nx = d0.nx;
+        L.p2_tk = d0.tk;
         L.xv = d0.xv;
         L.fft_f32 = d0.fft_f32;
         L.uniform_phase = 1;

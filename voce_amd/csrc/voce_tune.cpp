@@ -139,9 +139,9 @@ int measure(voce_ctx *c, voce_tune_choice *best)
         }
     // x pass: variant x tile x radix set
     const int tks[5] = {1, 2, 4, 8, 16};
-    for (int xv = 0; xv < 3 && !rc; ++xv)
+    for (int xv = 0; xv < VOCE_N_XV && !rc; ++xv)
         for (int ti = 0; ti < 5 && !rc; ++ti)
-            for (int r8 = 1; r8 >= 0 && !rc; --r8) {
+            for (int r8 = 1; r8 >= (xv == 3 ? 1 : 0) && !rc; --r8) {   // xv 3 has fixed radices
                 cand = keep;
                 cand.xv = xv;
                 cand.tk = tks[ti];
