@@ -476,6 +476,16 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         }
     }
     if (su->n_phases == 1) { pcs[1] = pcs[0]; pcs[1].gas = 1; pcs[1].nsys = 0; }
+    // the Newton kernel's flow rule skips the exactly-zero Schmid components
+    // of the standard FCC slip systems (the skipped terms add exact zeros)
+    for (int p = 0; p < 2; ++p) {
+        PhaseConst &pc = pcs[p];
+        int ok = !pc.gas && pc.nsys == 12;
+        for (int sy = 0; ok && sy < 12; ++sy)
+            for (int i = 0; i < 5; ++i)
+                if ((pc.px[sy][i] != 0.0) != (((voce_fcc_mask(sy) >> i) & 1u) != 0)) ok = 0;
+        pc.fcc12 = ok;
+    }
 
     // orientations (InitMicroStruct, init.c:393-483), one table row per
     // distinct (phase, Euler triple)

@@ -49,6 +49,9 @@ struct PhaseConst {
     int gas;
     int two_crss;   // 1 when forward and backward CRSS are stored separately
     int int_n;      // 1 when nsrs is an integer in [1, 64]
+    int fcc12;      // 1 when the twelve Schmid vectors have the zero pattern of the
+                    // {111}<110> systems in the order of Cu_p.sx (see voce_fcc_mask)
+    int pad_;
     double nsr, gam0, tau0f, tau0b, tau1, thet0, thet1;
     double px[12][5];      // crystal-frame Schmid vectors Schm_xt (init.c:217-224)
     double p3[12][6];      // the same as symmetric tensors: 00 11 22 12 02 01
@@ -168,6 +171,10 @@ int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_de
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
 #define VOCE_N_PLV 5                // plane-kernel variants (4: line-parallel square power-of-two planes)
+// nonzero B-basis components (bit a) of the crystal-frame Schmid vectors of
+// the twelve {111}<110> systems in the order of the reference's Cu_p.sx
+// (three systems per plane)
+static inline unsigned voce_fcc_mask(int s) { return (s % 3 == 0) ? 27u : (s % 3 == 1) ? 23u : 13u; }
 #define VOCE_N_XV 4                 // x-pass variants (3: line-parallel, Nx 16, 32, 64)
 int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);

@@ -1707,7 +1707,9 @@ enum { PM_N20 = 0, PM_INT = 1, PM_REAL = 2 };
 // strain rate of the last flow-rule evaluation), lds_old (the stress it was
 // evaluated at), lds_w (crystal-frame w), lds_ic (1 / CRSS).  Returns 0, or 35 (the reference's singular-matrix code) for a
 // non-positive pivot.
-template <int NT, bool TWO, int PM>
+template <unsigned V> struct UMask { static constexpr unsigned value = V; };
+
+template <int NT, bool TWO, int PM, bool FCC>
 __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double signorm, cptr<PhaseConst> pc, int nsys,
                                               double dt, gptr<double> kcv, double (*lds_w)[NT], double (*lds_old)[NT],
                                               double (*lds_ic)[NT])
@@ -1766,7 +1768,78 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
                 for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
             }
         };
+        // the standard FCC systems: components that are exactly zero (the
+        // pattern voce_fcc_mask gives, checked on the host) are left out of
+        // the sums, which is exact
+        auto rates_m = [&](auto Mt, const double *P, double icf, double icb, double &rss1, double &rss2) {
+            constexpr unsigned M = decltype(Mt)::value;
+            double rss0 = 0.0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a)
+                if ((M >> a) & 1u) rss0 += P[a] * sig[a];
+            const double itc = (rss0 < 0) ? icb : icf;
+            const double rr = rss0 * itc;
+            double pn1, pn;
+            if (PM == PM_N20) {
+                const double ar = fabs(rr);
+                const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+                pn1 = (ar * x2) * x16;
+                pn = pn1 * ar;
+            } else if (PM == PM_INT) {
+                const double ar = fabs(rr);
+                pn1 = ipow(ar, ni - 1);
+                pn = pn1 * ar;
+            } else {
+                pn1 = fabs(pow(rr, nsr - 1.));
+                pn = fabs(pow(rr, nsr));
+            }
+            rss1 = g0 * nsr * pn1 * itc;
+            rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
+        };
+        auto accum_m = [&](auto Mt, const double *P, double rss1, double rss2) {
+            constexpr unsigned M = decltype(Mt)::value;
+#pragma unroll
+            for (int a = 0; a < 5; ++a)
+                if ((M >> a) & 1u) ed[a] += P[a] * rss2;
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const double t = P[a] * rss1;
+#pragma unroll
+                for (int b2 = a; b2 < 5; ++b2) {
+                    if (((M >> a) & (M >> b2) & 1u) != 0) D[q] += t * P[b2];
+                    ++q;
+                }
+            }
+        };
+        auto pair_m = [&](int s, auto M0, auto M1) {
+            RELOAD_BARRIER();
+            double P0[5], P1[5];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                P0[a] = pc->px[s][a];
+                P1[a] = pc->px[s + 1][a];
+            }
+            const double f0 = lds_ic[s][threadIdx.x], f1 = lds_ic[s + 1][threadIdx.x];
+            const double b0 = TWO ? lds_ic[TWO ? 12 + s : 0][threadIdx.x] : f0;
+            const double b1 = TWO ? lds_ic[TWO ? 13 + s : 0][threadIdx.x] : f1;
+            double r10, r20, r11, r21;
+            rates_m(M0, P0, f0, b0, r10, r20);
+            rates_m(M1, P1, f1, b1, r11, r21);
+            accum_m(M0, P0, r10, r20);
+            accum_m(M1, P1, r11, r21);
+        };
         int s = 0;
+        if (FCC) {
+            // masks 27, 23, 13 per plane (voce_fcc_mask), two systems per trip
+            pair_m(0, UMask<27u>{}, UMask<23u>{});
+            pair_m(2, UMask<13u>{}, UMask<27u>{});
+            pair_m(4, UMask<23u>{}, UMask<13u>{});
+            pair_m(6, UMask<27u>{}, UMask<23u>{});
+            pair_m(8, UMask<13u>{}, UMask<27u>{});
+            pair_m(10, UMask<23u>{}, UMask<13u>{});
+            s = 12;
+        }
         for (; s + 1 < nsys; s += 2) {
             RELOAD_BARRIER();   // Schmid vectors are re-read from the scalar cache
             double P0[5], P1[5];
@@ -1940,12 +2013,14 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             const int ni = (int)pc->nsr;
             const int pm = pc->int_n ? (ni == 20 ? PM_N20 : PM_INT) : PM_REAL;
             int err;
-            if (pm == PM_N20)
-                err = newton_crystal<NT, TWO, PM_N20>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+            if (pm == PM_N20 && pc->fcc12)
+                err = newton_crystal<NT, TWO, PM_N20, true>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+            else if (pm == PM_N20)
+                err = newton_crystal<NT, TWO, PM_N20, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else if (pm == PM_INT)
-                err = newton_crystal<NT, TWO, PM_INT>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_INT, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else
-                err = newton_crystal<NT, TWO, PM_REAL>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_REAL, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             if (err) ms->error = err;
             // back to the sample frame: stress, the stress of the last
             // flow-rule evaluation and its strain rate
