@@ -28,6 +28,16 @@ def test_config2_64cube_against_oracle(tmp_path, tuning):
     compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "svm", "evm"), tuning=tuning)
 
 
+def test_80cube_radix5_against_oracle(tmp_path):
+    # a radix-5 grid at full size (80 = 16 x 5): the 64^3 case's material and
+    # load on 80^3, IterMax cut to 10 so the oracle finishes in seconds
+    c = inputs.config_case(2)
+    c.dims, c.n_grains, c.seed = (80, 80, 80), 125, 7
+    c.iter_max = 10
+    c.n_steps = 2
+    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad"))
+
+
 def _config3(tmp_path):
     c = inputs.config_case(3)
     d = str(tmp_path)

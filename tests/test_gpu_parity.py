@@ -214,3 +214,31 @@ def test_mixed_line_parallel_planes_against_oracle(tmp_path):
         say("incr", k + 1, "mixed (plane variant 4) vs oracle", r)
         assert r < 1e-4
     assert rel(gf["sig"], of["sig"]) < 1e-6
+
+
+@pytest.mark.parametrize("dims,n", [((10, 10, 10), 3), ((20, 20, 20), 2), ((15, 9, 7), 2), ((14, 21, 10), 2),
+                                    ((5, 7, 25), 2)])
+def test_fp64_parity_radix5_radix7_odd_planes(tmp_path, dims, n):
+    # sizes with factors 5 and 7 and odd Ny / Nz (an odd plane packs its last
+    # row with a row of zeros), against the oracle
+    c = inputs.Case(dims=dims, n_grains=5, seed=29, n_steps=n)
+    compare(tmp_path, c, n)
+
+
+def test_fp64_parity_9cube_elastic_converges(tmp_path):
+    # all-odd grid: no Nyquist planes, the elastic iteration converges early
+    c = inputs.Case(dims=(9, 9, 9), n_grains=4, seed=3, n_steps=2, material=inputs.Material(tau0f=1e4, tau0b=1e4))
+    g, o = compare(tmp_path, c, 2, fields=("sig", "disgrad", "tranmat", "crss", "svm", "evm"))
+    assert [r["iter"] for r in g] == [34, 32]
+
+
+def test_mixed_radix5_against_oracle(tmp_path):
+    c = inputs.Case(dims=(20, 20, 20), n_grains=8, seed=31, n_steps=2)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",))
+    o, of = oracle_run(path, base, 2, fields=("sig",))
+    for k in range(2):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed 20^3 vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-6

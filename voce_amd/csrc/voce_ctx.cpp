@@ -326,7 +326,7 @@ extern "C" const char *voce_strerror(int code)
     case VOCE_ERR_MESH_INDEX: return "inconsistent index in the microstructure file (reference abort 1139)";
     case VOCE_ERR_ARG: return "invalid argument";
     case VOCE_ERR_FILE: return "cannot open or parse an input file";
-    case VOCE_ERR_UNSUPPORTED_GRID: return "grid size not supported (each dimension 2^a 3^b, even, 2..128)";
+    case VOCE_ERR_UNSUPPORTED_GRID: return "grid size not supported (each dimension 2^a 3^b 5^c 7^d, 2..128)";
     case VOCE_ERR_DEVICE: return "HIP runtime error";
     case VOCE_ERR_NOMEM: return "device memory allocation failed";
     case VOCE_ERR_NOT_LOADED: return "HIP library not loaded";
@@ -424,7 +424,6 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     const int nx = su->dims[0], ny = su->dims[1], nz = su->dims[2];
     if (nx <= 0 || ny <= 0 || nz <= 0 || su->n_phases < 1 || su->n_phases > 2) return VOCE_ERR_ARG;
     if (!voce_grid_supported(nx) || !voce_grid_supported(ny) || !voce_grid_supported(nz)) return VOCE_ERR_UNSUPPORTED_GRID;
-    if (ny % 2 || nz % 2) return VOCE_ERR_UNSUPPORTED_GRID;   // the plane kernels pack row pairs
     if (su->iter_max < 0) return VOCE_ERR_ARG;
     const long V = (long)nx * ny * nz;
     const double WGT = 1. / V;
@@ -706,7 +705,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;
     // default FFT launch choices (a tuned set replaces them)
-    d.plv = (ny * d.nzh <= 2112) ? 1 : 2;
+    d.plv = voce_plane_lds(&d, 1) > 0 ? 1 : 2;
     if (voce_plane_lds(&d, 4) > 0) d.plv = 4;   // line-parallel planes when the grid allows them
     d.xv = 0;
     d.tk = voce_xgreen_tile(nx, 0);

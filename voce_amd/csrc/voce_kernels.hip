@@ -168,6 +168,67 @@ template <bool INV, typename C> struct Bfly<8, INV, C> {
     }
 };
 
+// Odd-prime butterflies (R = 5, 7) in the symmetric form: with
+// t_j = v_j + v_(R-j), u_j = v_j - v_(R-j) (j = 1..(R-1)/2),
+// X_0 = v_0 + sum t_j, and for k = 1..(R-1)/2
+// X_k = m_k -/+ i n_k, X_(R-k) = m_k +/- i n_k (forward/inverse) with
+// m_k = v_0 + sum_j cos(2 pi j k / R) t_j, n_k = sum_j sin(2 pi j k / R) u_j.
+template <int R> struct OddTrig;
+template <> struct OddTrig<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494742, -0.80901699437494742, -0.80901699437494742,
+                                    0.30901699437494742};
+    static constexpr double s[5] = {0.0, 0.95105651629515357, 0.58778525229247313, -0.58778525229247313,
+                                    -0.95105651629515357};
+};
+template <> struct OddTrig<7> {
+    static constexpr double c[7] = {1.0, 0.62348980185873353, -0.22252093395631440, -0.90096886790241913,
+                                    -0.90096886790241913, -0.22252093395631440, 0.62348980185873353};
+    static constexpr double s[7] = {0.0, 0.78183148246802981, 0.97492791218182361, 0.43388373911755812,
+                                    -0.43388373911755812, -0.97492791218182361, -0.78183148246802981};
+};
+
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void bfly_odd(C *v)
+{
+    using T = decltype(v[0].x);
+    constexpr int H = (R - 1) / 2;
+    C t[H + 1], u[H + 1];
+    C x0 = v[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        t[j] = cadd(v[j], v[R - j]);
+        u[j] = csub(v[j], v[R - j]);
+        x0 = cadd(x0, t[j]);
+    }
+    C out[R];
+    out[0] = x0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        C m = v[0], n = C{(T)0, (T)0};
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const T cj = (T)OddTrig<R>::c[(j * k) % R], sj = (T)OddTrig<R>::s[(j * k) % R];
+            m = C{m.x + cj * t[j].x, m.y + cj * t[j].y};
+            n = C{n.x + sj * u[j].x, n.y + sj * u[j].y};
+        }
+        if (!INV) {
+            out[k] = C{m.x + n.y, m.y - n.x};
+            out[R - k] = C{m.x - n.y, m.y + n.x};
+        } else {
+            out[k] = C{m.x - n.y, m.y + n.x};
+            out[R - k] = C{m.x + n.y, m.y - n.x};
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+}
+template <bool INV, typename C> struct Bfly<5, INV, C> {
+    __device__ __forceinline__ static void run(C *v) { bfly_odd<5, INV, C>(v); }
+};
+template <bool INV, typename C> struct Bfly<7, INV, C> {
+    __device__ __forceinline__ static void run(C *v) { bfly_odd<7, INV, C>(v); }
+};
+
 // Index arithmetic of the FFT kernels.  Every index there is below 2^22,
 // so products use the full-rate 24-bit multiply, and a / d comes from a
 // float reciprocal with one correction step ((float)a is exact and a * (1/d)
@@ -251,8 +312,8 @@ __device__ __forceinline__ void fft_stage(C *s, int nlines, int N, int Ns, int l
     __syncthreads();
 }
 
-// Mixed-radix plan: radix-3 stages, then radix 8/4/2 for the power of two
-// (r8 = 0: radix 4/2 only; the choice is tuned per grid size).
+// Mixed-radix plan: radix-3, -5 and -7 stages, then radix 8/4/2 for the
+// power of two (r8 = 0: radix 4/2 only; the choice is tuned per grid size).
 template <typename C, int NT, int MAXEL, bool INV, bool LF>
 __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, const C *tw, int r8)
 {
@@ -261,6 +322,16 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
         fft_stage<C, 3, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw);
         Ns *= 3;
         rem /= 3;
+    }
+    while (rem % 5 == 0) {
+        fft_stage<C, 5, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw);
+        Ns *= 5;
+        rem /= 5;
+    }
+    while (rem % 7 == 0) {
+        fft_stage<C, 7, NT, MAXEL, INV, LF>(s, nlines, N, Ns, lstride, estride, tw);
+        Ns *= 7;
+        rem /= 7;
     }
     int a = 0;
     while ((1 << a) < rem) ++a;
@@ -327,7 +398,9 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
     const int RS = 2 * nzh;   // packed z rows: two real rows as one complex row
-    C *twy = s + ny * nzh;
+    // an odd last row is packed with a row of zeros
+    const int np = (ny + 1) >> 1;
+    C *twy = s + 2 * np * nzh;
     C *twz = twy + ny;
     const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const C *gty = (const C *)r.tw_y;
@@ -342,13 +415,15 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
         const int y = bynz.div(e, z);
         sr[2 * ((y >> 1) * RS + z) + (y & 1)] = (T)v;
     });
+    if (ny & 1)
+        for (int z = tid; z < nz; z += NT) sr[2 * ((np - 1) * RS + z) + 1] = (T)0;
     __syncthreads();
-    fft_lines<C, NT, MAXEL, false, false>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
+    fft_lines<C, NT, MAXEL, false, false>(s, np, nz, RS, 1, twz, r.pl_r8);
     // split packed rows a + i b into the half spectra of a and b
     {
         constexpr int SPT = (MAXEL / 2 + NT - 1) / NT;
         C A[SPT], B[SPT];
-        const int nit = (ny / 2) * nzh;
+        const int nit = np * nzh;
 #pragma unroll
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
@@ -370,7 +445,7 @@ __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rta
                 int k;
                 const int p = bynzh.div(it, k);
                 s[(2 * p) * nzh + k] = A[i];
-                s[(2 * p + 1) * nzh + k] = B[i];
+                if (2 * p + 1 < ny) s[(2 * p + 1) * nzh + k] = B[i];
             }
         }
         __syncthreads();
@@ -397,7 +472,8 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     C *s = (C *)smem;
     const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh;
     const int RS = 2 * nzh;   // packed z rows, as in k_plane_fwd
-    C *twy = s + ny * nzh;
+    const int np = (ny + 1) >> 1;
+    C *twy = s + 2 * np * nzh;
     C *twz = twy + ny;
     const int c = lb / nx, x = lb - c * nx, tid = threadIdx.x;
     const FastDiv bynz(nz);
@@ -416,7 +492,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     {
         constexpr int SPT = (MAXEL + NT - 1) / NT;
         C Z[SPT];
-        const int nit = (ny / 2) * nz;
+        const int nit = np * nz;
 #pragma unroll
         for (int i = 0; i < SPT; ++i) {
             const int it = tid + i * NT;
@@ -424,13 +500,14 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
                 int k;
                 const int p = bynz.div(it, k);
                 C A, B;
+                const bool hb = 2 * p + 1 < ny;   // an odd last row pairs with zeros
                 if (k <= nz / 2) {
                     A = s[(2 * p) * nzh + k];
-                    B = s[(2 * p + 1) * nzh + k];
+                    B = hb ? s[(2 * p + 1) * nzh + k] : C{(T)0, (T)0};
                     if (k == 0 || 2 * k == nz) { A.y = (T)0; B.y = (T)0; }
                 } else {
                     A = s[(2 * p) * nzh + nz - k];
-                    B = s[(2 * p + 1) * nzh + nz - k];
+                    B = hb ? s[(2 * p + 1) * nzh + nz - k] : C{(T)0, (T)0};
                     A.y = -A.y;
                     B.y = -B.y;
                 }
@@ -449,7 +526,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
         }
         __syncthreads();
     }
-    fft_lines<C, NT, MAXEL, true, false>(s, ny / 2, nz, RS, 1, twz, r.pl_r8);
+    fft_lines<C, NT, MAXEL, true, false>(s, np, nz, RS, 1, twz, r.pl_r8);
     const T *sr = (const T *)s;
     const long off = (long)c * r.V + (long)x * ny * nz;
     if (mode == 0) {
@@ -2697,13 +2774,15 @@ extern "C" int voce_kernels_prepare(void)
     return (int)e;
 }
 
-// 2^a 3^b, 2..128 (y and z must also be even: the plane kernels pack row pairs)
+// 2^a 3^b 5^c 7^d, 2..128
 extern "C" int voce_grid_supported(int n)
 {
     if (n < 2 || n > 128) return 0;
     int m = n;
     while (m % 2 == 0) m /= 2;
     while (m % 3 == 0) m /= 3;
+    while (m % 5 == 0) m /= 5;
+    while (m % 7 == 0) m /= 7;
     return m == 1;
 }
 
@@ -2742,8 +2821,10 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
         const size_t b = (size_t)el * cb;
         return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
     }
-    if (plv < 0 || plv > 3 || el > PL_MAXEL[plv]) return 0;
-    const size_t b = (size_t)(el + r->ny + r->nz) * cb;
+    // an odd Ny packs its last row with a row of zeros: one more row
+    const int elp = (r->ny + (r->ny & 1)) * r->nzh;
+    if (plv < 0 || plv > 3 || elp > PL_MAXEL[plv]) return 0;
+    const size_t b = (size_t)(elp + r->ny + r->nz) * cb;
     return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
 }
 
