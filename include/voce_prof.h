@@ -3,10 +3,12 @@
  *
  * No counterpart in the reference (it has no timers, see evolution.c:344-348
  * for its only progress output).  For benchmarks that must report a
- * kernel's own average duration: HIP events are recorded on the context's
- * stream between the launches of every k-th FFT iteration (k chosen at
- * enable; each recorded event costs a few microseconds of queue time, so a
- * sparse sample keeps the timed run close to an untimed one).  The sampled
+ * kernel's own average duration: each kernel of every k-th FFT iteration is
+ * launched with a pair of HIP events bound to its own dispatch
+ * (hipExtLaunchKernelGGL start/stop events), so a sample is the kernel's
+ * execution from its start to its end, without launch gaps (k chosen at
+ * enable; a profiled dispatch costs a little queue time, so a sparse
+ * sample keeps the timed run close to an untimed one).  The sampled
  * residue shifts by one per increment, so k consecutive increments sample
  * every iteration index once.  Only iterations that did work (before
  * convergence) are accumulated.

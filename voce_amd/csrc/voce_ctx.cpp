@@ -893,9 +893,9 @@ int voce_run_increment(voce_ctx *c, voce_incr_report *rep, int timing)
     while (launched < imax && c->h_ctl[3]) {
         int n = chunk;
         if (launched + n > imax) n = imax - launched;
-        if (timing && (int)c->ev.size() < 6 * n) {
+        if (timing && (int)c->ev.size() < 8 * n) {
             const size_t old = c->ev.size();
-            c->ev.resize(6 * n);
+            c->ev.resize(8 * n);
             for (size_t i = old; i < c->ev.size(); ++i) CK(hipEventCreate(&c->ev[i]));
         }
         // timed iterations: every prof-th FFT iteration, the residue moving
@@ -903,7 +903,7 @@ int voce_run_increment(voce_ctx *c, voce_incr_report *rep, int timing)
         // Newton steps; over prof increments every iteration index is sampled)
         auto timed = [&](int it) { return timing > 0 && (it + c->n_steps_done) % timing == 0; };
         for (int i = 0; i < n; ++i) {
-            e = voce_launch_iteration(&c->L, c->stream, timed(launched + i) ? (void *)&c->ev[6 * i] : nullptr);
+            e = voce_launch_iteration(&c->L, c->stream, timed(launched + i) ? (void *)&c->ev[8 * i] : nullptr);
             if (e) return VOCE_ERR_DEVICE;
         }
         launched += n;
@@ -917,9 +917,9 @@ int voce_run_increment(voce_ctx *c, voce_incr_report *rep, int timing)
             const int ran = c->h_ctl[1] - done_before;
             for (int i = 0; i < ran && i < n; ++i) {
                 if (!timed(launched - n + i)) continue;
-                for (int k = 0; k < VOCE_PROF_FIN; ++k) {   // events ev[0..4] bracket the 4 kernels
+                for (int k = 0; k < VOCE_PROF_FIN; ++k) {   // start/end pair of kernel k
                     float ms = 0.f;
-                    if (hipEventElapsedTime(&ms, c->ev[6 * i + k], c->ev[6 * i + k + 1]) == hipSuccess) {
+                    if (hipEventElapsedTime(&ms, c->ev[8 * i + 2 * k], c->ev[8 * i + 2 * k + 1]) == hipSuccess) {
                         c->prof_ms[k] += ms;
                         c->prof_n[k] += 1;
                     }

@@ -37,7 +37,7 @@ struct voce_ctx {
     size_t arena_bytes;    // device bytes of the context's arena
     // kernel timing (include/voce_prof.h)
     int prof;
-    std::vector<hipEvent_t> ev;    // 6 per enqueued iteration
+    std::vector<hipEvent_t> ev;    // 8 per enqueued iteration (start/end per kernel)
     double prof_ms[VOCE_PROF_NK];
     long prof_n[VOCE_PROF_NK];
 };

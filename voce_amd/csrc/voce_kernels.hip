@@ -31,6 +31,7 @@
 // FFT type T is double (FP64 mode) or float (MIXED mode); the constitutive
 // update, the reductions and all state stay double.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <math.h>
 #include "voce_internal.h"
 
@@ -2637,21 +2638,33 @@ constexpr int NT_X = 256, MAXEL_X = 3072;
 constexpr int NT_XB = 128, MAXEL_XB = 1536;
 constexpr int NT_XC = 64, MAXEL_XC = 768;
 
+// Events the next iteration-kernel launch binds to its own dispatch
+// (hipExtLaunchKernelGGL): the first takes the kernel's start time, the
+// second its end, so their elapsed time is the kernel's execution without
+// the queue's launch gaps.  Null outside timed launches, which then go
+// through the plain launch (graph capture included).
+thread_local hipEvent_t g_kt0 = nullptr, g_kt1 = nullptr;
+#define VLAUNCH(kern, grid, block, shm, st, ...)                                           \
+    do {                                                                                    \
+        if (g_kt0) hipExtLaunchKernelGGL(kern, grid, block, shm, st, g_kt0, g_kt1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                   \
+    } while (0)
+
 template <typename T>
 void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
 {
     switch (L->plv) {
-    case 0: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_pfwd), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_pfwd); break;
-    case 1: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_pfwd), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_pfwd); break;
-    case 3: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>), dim3(L->n_pfwd), dim3(NT_PL_M), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 0: VLAUNCH((k_plane_fwd<T, NT_PL_T, MAXEL_PL_T>), dim3(L->n_pfwd), dim3(NT_PL_T), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 1: VLAUNCH((k_plane_fwd<T, NT_PL_S, MAXEL_PL_S>), dim3(L->n_pfwd), dim3(NT_PL_S), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    case 3: VLAUNCH((k_plane_fwd<T, NT_PL_M, MAXEL_PL_M>), dim3(L->n_pfwd), dim3(NT_PL_M), L->pl_lds, st, L->tab, L->map_pfwd); break;
     case 4: {   // line-parallel, square power-of-two planes (the grid's ny)
         const int n = L->p2_n;
-        if (n == 16) hipLaunchKernelGGL((k_plane_fwd_p2<T, 16>), dim3(L->n_pfwd), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
-        else if (n == 32) hipLaunchKernelGGL((k_plane_fwd_p2<T, 32>), dim3(L->n_pfwd), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
-        else hipLaunchKernelGGL((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        if (n == 16) VLAUNCH((k_plane_fwd_p2<T, 16>), dim3(L->n_pfwd), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else if (n == 32) VLAUNCH((k_plane_fwd_p2<T, 32>), dim3(L->n_pfwd), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         break;
     }
-    default: hipLaunchKernelGGL((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
+    default: VLAUNCH((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
 
@@ -2661,17 +2674,17 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
     const int n = mode ? L->n_plane3 : L->n_plane6;
     const int2 *map = mode ? L->map_plane3 : L->map_plane6;
     switch (L->plv) {
-    case 0: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_T, MAXEL_PL_T>), dim3(n), dim3(NT_PL_T), L->pl_lds, st, L->tab, map, mode); break;
-    case 1: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(n), dim3(NT_PL_S), L->pl_lds, st, L->tab, map, mode); break;
-    case 3: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_M, MAXEL_PL_M>), dim3(n), dim3(NT_PL_M), L->pl_lds, st, L->tab, map, mode); break;
+    case 0: VLAUNCH((k_plane_inv<T, NT_PL_T, MAXEL_PL_T>), dim3(n), dim3(NT_PL_T), L->pl_lds, st, L->tab, map, mode); break;
+    case 1: VLAUNCH((k_plane_inv<T, NT_PL_S, MAXEL_PL_S>), dim3(n), dim3(NT_PL_S), L->pl_lds, st, L->tab, map, mode); break;
+    case 3: VLAUNCH((k_plane_inv<T, NT_PL_M, MAXEL_PL_M>), dim3(n), dim3(NT_PL_M), L->pl_lds, st, L->tab, map, mode); break;
     case 4: {
         const int m = L->p2_n;
-        if (m == 16) hipLaunchKernelGGL((k_plane_inv_p2<T, 16>), dim3(n), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, map, mode);
-        else if (m == 32) hipLaunchKernelGGL((k_plane_inv_p2<T, 32>), dim3(n), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, map, mode);
-        else hipLaunchKernelGGL((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
+        if (m == 16) VLAUNCH((k_plane_inv_p2<T, 16>), dim3(n), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, map, mode);
+        else if (m == 32) VLAUNCH((k_plane_inv_p2<T, 32>), dim3(n), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, map, mode);
+        else VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
         break;
     }
-    default: hipLaunchKernelGGL((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
+    default: VLAUNCH((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
     }
 }
 
@@ -2679,8 +2692,8 @@ template <typename T, int NX>
 void launch_xgreen_p2(const LaunchSet *L, hipStream_t st, int mode)
 {
     const int wpb = L->p2_tk / XP2<NX>::CPW;
-    if (wpb == 4) hipLaunchKernelGGL((k_xgreen_p2<T, NX, 4>), dim3(L->n_x), dim3(256), L->x_lds, st, L->tab, L->map_x, mode);
-    else hipLaunchKernelGGL((k_xgreen_p2<T, NX, 2>), dim3(L->n_x), dim3(128), L->x_lds, st, L->tab, L->map_x, mode);
+    if (wpb == 4) VLAUNCH((k_xgreen_p2<T, NX, 4>), dim3(L->n_x), dim3(256), L->x_lds, st, L->tab, L->map_x, mode);
+    else VLAUNCH((k_xgreen_p2<T, NX, 2>), dim3(L->n_x), dim3(128), L->x_lds, st, L->tab, L->map_x, mode);
 }
 
 template <typename T>
@@ -2692,41 +2705,40 @@ void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
         else if (L->p2_nx == 32) launch_xgreen_p2<T, 32>(L, st, mode);
         else launch_xgreen_p2<T, 64>(L, st, mode);
         break;
-    case 0: hipLaunchKernelGGL((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
-    case 1: hipLaunchKernelGGL((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
-    default: hipLaunchKernelGGL((k_xgreen<T, NT_XC, MAXEL_XC>), dim3(L->n_x), dim3(NT_XC), L->x_lds, st, L->tab, L->map_x, mode); break;
+    case 0: VLAUNCH((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
+    case 1: VLAUNCH((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
+    default: VLAUNCH((k_xgreen<T, NT_XC, MAXEL_XC>), dim3(L->n_x), dim3(NT_XC), L->x_lds, st, L->tab, L->map_x, mode); break;
     }
 }
 
 void launch_nr(const LaunchSet *L, hipStream_t st)
 {
     if (L->uniform_phase) {
-        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, true, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, true, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        if (L->two_crss) VLAUNCH((k_nr<NT_VOX, true, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else VLAUNCH((k_nr<NT_VOX, true, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     } else {
-        if (L->two_crss) hipLaunchKernelGGL((k_nr<NT_VOX, false, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
-        else hipLaunchKernelGGL((k_nr<NT_VOX, false, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        if (L->two_crss) VLAUNCH((k_nr<NT_VOX, false, true>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
+        else VLAUNCH((k_nr<NT_VOX, false, false>), dim3(L->n_vox), dim3(NT_VOX), 0, st, L->tab, L->map_vox);
     }
 }
 
 template <typename T>
 hipError_t launch_iter_t(const LaunchSet *L, hipStream_t st, hipEvent_t *ev)
 {
-    int evi = 0;
-#define MARK()                                                   \
-    do {                                                         \
-        if (ev) (void)hipEventRecord(ev[evi++], st);             \
-    } while (0)
-    MARK();
+    // ev: null, or 8 events, a start/end pair per kernel in launch order
+    auto bind = [&](int k) {
+        g_kt0 = ev ? ev[2 * k] : nullptr;
+        g_kt1 = ev ? ev[2 * k + 1] : nullptr;
+    };
+    bind(0);
     launch_plane_fwd<T>(L, st);
-    MARK();
+    bind(1);
     launch_xgreen<T>(L, st, 0);
-    MARK();
+    bind(2);
     launch_plane_inv<T>(L, st, 0);
-    MARK();
+    bind(3);
     launch_nr(L, st);
-    MARK();
-#undef MARK
+    g_kt0 = g_kt1 = nullptr;
     return hipGetLastError();
 }
 
