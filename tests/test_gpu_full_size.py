@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("tuning", [{"plv": 1, "xv": 0, "tk": 4}, {"plv": 4, "xv": 3, "tk": 2},
-                                    {"plv": 4, "xv": 3, "tk": 4}])
+                                    {"plv": 4, "xv": 3, "tk": 4}, {"plv": 4, "xv": 4, "tk": 8},
+                                    {"plv": 4, "xv": 4, "tk": 4}, {"plv": 4, "xv": 4, "tk": 2}])
 def test_config2_64cube_against_oracle(tmp_path, tuning):
     c = inputs.config_case(2)
     c.iter_max = 10

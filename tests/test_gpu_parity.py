@@ -195,19 +195,31 @@ def test_mixed_precision_within_1e4(tmp_path):
                                          ((16, 16, 16), {"plv": 4, "xv": 3, "tk": 4}),
                                          ((16, 16, 16), {"plv": 4, "xv": 3, "tk": 8}),
                                          ((32, 12, 16), {"xv": 3, "tk": 4}),
-                                         ((32, 32, 32), {"plv": 4, "xv": 3, "tk": 8})])
+                                         ((32, 32, 32), {"plv": 4, "xv": 3, "tk": 8}),
+                                         ((16, 16, 16), {"plv": 4, "xv": 4, "tk": 8}),
+                                         ((16, 16, 16), {"xv": 4, "tk": 4}),
+                                         ((32, 12, 16), {"xv": 4, "tk": 8}),
+                                         ((32, 12, 16), {"xv": 4, "tk": 4}),
+                                         ((16, 16, 16), {"xv": 4, "tk": 2}),
+                                         ((32, 12, 16), {"xv": 4, "tk": 2}),
+                                         ((32, 32, 32), {"plv": 4, "xv": 4, "tk": 8})])
 def test_fp64_parity_line_parallel(tmp_path, dims, tuning):
     # plane variant 4 (line-parallel kernels of square power-of-two planes,
-    # any Nx) and x-pass variant 3 (line-parallel x pass, Nx = 16, 32, 64, any
-    # plane), against the oracle
+    # any Nx), x-pass variant 3 (line-parallel x pass, Nx = 16, 32, 64, any
+    # plane) and x-pass variant 4 (staged through LDS, a whole, half or
+    # quarter column block per block; 12 x 9 = 108 columns leave a partial
+    # last block),
+    # against the oracle
     c = inputs.Case(dims=dims, n_grains=6, seed=19, n_steps=3)
     compare(tmp_path, c, 3, tuning=tuning)
 
 
-def test_mixed_line_parallel_planes_against_oracle(tmp_path):
+@pytest.mark.parametrize("tuning", [{"plv": 4, "xv": 3, "tk": 4}, {"plv": 4, "xv": 4, "tk": 16},
+                                    {"plv": 4, "xv": 4, "tk": 8}, {"plv": 4, "xv": 4, "tk": 4}])
+def test_mixed_line_parallel_planes_against_oracle(tmp_path, tuning):
     c = inputs.Case(dims=(32, 32, 32), n_grains=8, seed=23, n_steps=3)
     path, base = write(tmp_path, c)
-    g, gf = gpu_run(path, base, 3, "mixed", fields=("sig",), tuning={"plv": 4, "xv": 3, "tk": 4})
+    g, gf = gpu_run(path, base, 3, "mixed", fields=("sig",), tuning=tuning)
     o, of = oracle_run(path, base, 3, fields=("sig",))
     for k in range(3):
         r = rel(g[k]["sig_avg"], o[k]["sig_avg"])

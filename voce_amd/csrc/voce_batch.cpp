@@ -244,7 +244,7 @@ static int tune_group_timed(voce_batch *b, Group &g)
     const int tks[5] = {1, 2, 4, 8, 16};
     for (int xv = 0; xv < VOCE_N_XV && !rc; ++xv)
         for (int ti = 0; ti < 5 && !rc; ++ti)
-            for (int r8 = 1; r8 >= (xv == 3 ? 1 : 0) && !rc; --r8) {   // xv 3 has fixed radices
+            for (int r8 = 1; r8 >= (xv >= 3 ? 1 : 0) && !rc; --r8) {   // xv 3 and 4 have fixed radices
                 cand = keep;
                 cand.xv = xv;
                 cand.tk = tks[ti];

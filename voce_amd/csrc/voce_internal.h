@@ -175,7 +175,7 @@ int voce_grid_supported(int n);
 // the twelve {111}<110> systems in the order of the reference's Cu_p.sx
 // (three systems per plane)
 static inline unsigned voce_fcc_mask(int s) { return (s % 3 == 0) ? 27u : (s % 3 == 1) ? 23u : 13u; }
-#define VOCE_N_XV 4                 // x-pass variants (3: line-parallel, Nx 16, 32, 64)
+#define VOCE_N_XV 5                 // x-pass variants (3: line-parallel, 4: staged through LDS; Nx 16, 32, 64)
 int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);
 int voce_vox_block(void);   // threads per block of the per-voxel kernels

@@ -55,6 +55,24 @@ template <typename T> using gptr = const __attribute__((address_space(1))) T *;
 template <typename T> __device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)p; }
 template <typename T> using gmut = __attribute__((address_space(1))) T *;
 template <typename T> __device__ __forceinline__ gmut<T> as_global_mut(T *p) { return (gmut<T>)p; }
+// The same for the two-component vector types (complex elements, pairs of
+// doubles), through the native vector type of the same layout: the HIP
+// vector types only bind generic references.
+template <typename V2> struct vec2_of;
+template <> struct vec2_of<double2> { typedef double t __attribute__((ext_vector_type(2))); };
+template <> struct vec2_of<float2> { typedef float t __attribute__((ext_vector_type(2))); };
+template <typename V2> __device__ __forceinline__ V2 gload(const V2 *p)
+{
+    const typename vec2_of<V2>::t v = *(const __attribute__((address_space(1))) typename vec2_of<V2>::t *)p;
+    return V2{v.x, v.y};
+}
+template <typename V2> __device__ __forceinline__ void gstore(V2 *p, V2 a)
+{
+    typename vec2_of<V2>::t v;
+    v.x = a.x;
+    v.y = a.y;
+    *(__attribute__((address_space(1))) typename vec2_of<V2>::t *)p = v;
+}
 
 // Strided element loop e = tid, tid + NT, ... < n whose global loads go out
 // U at a time: v = ld(e) for a group of U elements, then st(e, v) for each.
@@ -652,7 +670,7 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__
     {
         const C *gtw = (const C *)r.tw_z;   // tw_y holds the same table (Ny = Nz)
 #pragma unroll
-        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
     }
     // z: packed row pair p = grp, two real rows as one complex row
     if (grp < N / 2) {
@@ -664,8 +682,8 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__
         double2 pa[R / 2], pb[R / 2];
 #pragma unroll
         for (int q = 0; q < R / 2; ++q) {
-            pa[q] = src[n1 + L * q];
-            pb[q] = src[N / 2 + n1 + L * q];
+            pa[q] = gload(src + n1 + L * q);
+            pb[q] = gload(src + N / 2 + n1 + L * q);
         }
 #pragma unroll
         for (int q = 0; q < R / 2; ++q) {
@@ -717,7 +735,7 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__
 #pragma unroll
         for (int o = 0; o < R; ++o) {
             const int e = line_out_index<N>(n1, o) * NZH + grp;
-            dst[((long)(e / TB) * nx + x) * TB + (e % TB)] = v[o];
+            gstore(dst + ((long)(e / TB) * nx + x) * TB + (e % TB), v[o]);
         }
     }
 }
@@ -748,15 +766,15 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
         const double2 *E = (const double2 *)(r.E6 + off);
 #pragma unroll
         for (int q = 0; q < R / 2; ++q) {
-            ev[0][q] = E[n1 + L * q];
-            ev[1][q] = E[N / 2 + n1 + L * q];
+            ev[0][q] = gload(E + n1 + L * q);
+            ev[1][q] = gload(E + N / 2 + n1 + L * q);
         }
     }
     C tw[R];
     {
         const C *gtw = (const C *)r.tw_z;
 #pragma unroll
-        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
     }
     // y: half-spectrum column kz = grp, inverse DFT along y
     if (grp < NZH) {
@@ -766,7 +784,7 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int e = (n1 + L * q) * NZH + grp;
-            v[q] = srcp[((long)(e / TB) * nx + x) * TB + (e % TB)];
+            v[q] = gload(srcp + ((long)(e / TB) * nx + x) * TB + (e % TB));
         }
         C *col = s + grp;
         line_dft<C, N, true>(v, tw, n1, [&](int i) { return col + i * NZH; });
@@ -818,11 +836,11 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
             for (int q = 0; q < R / 2; ++q) {
                 const int i = n1 + L * q;
                 if (creep_set) {
-                    E[i] = double2{cr + (sh + va[q].x), cr + (sh + va[q].y)};
-                    E[N / 2 + i] = double2{cr + (sh + vb[q].x), cr + (sh + vb[q].y)};
+                    gstore(E + i, double2{cr + (sh + va[q].x), cr + (sh + va[q].y)});
+                    gstore(E + N / 2 + i, double2{cr + (sh + vb[q].x), cr + (sh + vb[q].y)});
                 } else {
-                    E[i] = double2{ev[0][q].x + (sh + va[q].x), ev[0][q].y + (sh + va[q].y)};
-                    E[N / 2 + i] = double2{ev[1][q].x + (sh + vb[q].x), ev[1][q].y + (sh + vb[q].y)};
+                    gstore(E + i, double2{ev[0][q].x + (sh + va[q].x), ev[0][q].y + (sh + va[q].y)});
+                    gstore(E + N / 2 + i, double2{ev[1][q].x + (sh + vb[q].x), ev[1][q].y + (sh + vb[q].y)});
                 }
             }
             // diagnostics: the running sum of the Green updates (the
@@ -834,9 +852,9 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
 #pragma unroll
                 for (int q = 0; q < R / 2; ++q) {
                     const int i = n1 + L * q;
-                    const double2 fa = F[i], fb = F[N / 2 + i];
-                    F[i] = double2{fa.x + va[q].x, fa.y + va[q].y};
-                    F[N / 2 + i] = double2{fb.x + vb[q].x, fb.y + vb[q].y};
+                    const double2 fa = gload(F + i), fb = gload(F + N / 2 + i);
+                    gstore(F + i, double2{fa.x + va[q].x, fa.y + va[q].y});
+                    gstore(F + N / 2 + i, double2{fb.x + vb[q].x, fb.y + vb[q].y});
                 }
             }
         } else {
@@ -846,12 +864,12 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
 #pragma unroll
             for (int q = 0; q < R / 2; ++q) {
                 const int i = n1 + L * q;
-                W[i] = va[q];
-                W[N / 2 + i] = vb[q];
+                gstore(W + i, va[q]);
+                gstore(W + N / 2 + i, vb[q]);
                 if (F) {
-                    const double2 fa = F[i], fb = F[N / 2 + i];
-                    F[i] = double2{fa.x + va[q].x, fa.y + va[q].y};
-                    F[N / 2 + i] = double2{fb.x + vb[q].x, fb.y + vb[q].y};
+                    const double2 fa = gload(F + i), fb = gload(F + N / 2 + i);
+                    gstore(F + i, double2{fa.x + va[q].x, fa.y + va[q].y});
+                    gstore(F + N / 2 + i, double2{fb.x + vb[q].x, fb.y + vb[q].y});
                 }
             }
         }
@@ -1163,13 +1181,13 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
     C u[3];
     if (pt && !first) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) u[i] = usum[i * cstr + gidx(col2, kx)];
+        for (int i = 0; i < 3; ++i) u[i] = gload(usum + i * cstr + gidx(col2, kx));
     }
     C tw[R];
     {
         const C *gtw = (const C *)r.tw_x;
 #pragma unroll
-        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gtw[n1 * k1];
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
     }
     C *line = ws + (j * 6 + cc) * NX;
     C *dst = (C *)r.spec;
@@ -1178,17 +1196,17 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
         if (dft) {
             C v[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[q] = usum[cc * cstr + gidx(col, n1 + L * q)];
+            for (int q = 0; q < R; ++q) v[q] = gload(usum + cc * cstr + gidx(col, n1 + L * q));
             line_dft<C, NX, true>(v, tw, n1, [&](int i) { return line + i; });
 #pragma unroll
-            for (int o = 0; o < R; ++o) dst[cc * cstr + gidx(col, line_out_index<NX>(n1, o))] = v[o];
+            for (int o = 0; o < R; ++o) gstore(dst + cc * cstr + gidx(col, line_out_index<NX>(n1, o)), v[o]);
         }
         return;
     }
     if (dft) {
         C v[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[q] = dst[cc * cstr + gidx(col, n1 + L * q)];
+        for (int q = 0; q < R; ++q) v[q] = gload(dst + cc * cstr + gidx(col, n1 + L * q));
         line_dft<C, NX, false>(v, tw, n1, [&](int i) { return line + i; });
         wave_lds_sync();
 #pragma unroll
@@ -1230,7 +1248,7 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
         for (int c = 0; c < 6; ++c) pl[c * NX] = d[c];
         // sum of the skew parts over the increment (lattice spin)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) usum[i * cstr + gidx(col2, kx)] = first ? wk[i] : cadd(u[i], wk[i]);
+        for (int i = 0; i < 3; ++i) gstore(usum + i * cstr + gidx(col2, kx), first ? wk[i] : cadd(u[i], wk[i]));
     }
     wave_lds_sync();
     if (dft) {
@@ -1239,7 +1257,188 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
         for (int q = 0; q < R; ++q) v[q] = line[n1 + L * q];
         line_dft<C, NX, true>(v, tw, n1, [&](int i) { return line + i; });
 #pragma unroll
-        for (int o = 0; o < R; ++o) dst[cc * cstr + gidx(col, line_out_index<NX>(n1, o))] = v[o];
+        for (int o = 0; o < R; ++o) gstore(dst + cc * cstr + gidx(col, line_out_index<NX>(n1, o)), v[o]);
+    }
+}
+
+// ------------------------------------------------------------------
+// x pass staged through LDS (x-pass variant 4, Nx = 16, 32 or 64).  A block
+// owns TK columns of one 128-byte column block (all of its TB columns, a
+// half or a quarter): for each component the
+// block's elements (kx, column) form one contiguous run of Nx * TK complex
+// values, so the spectrum and skew-sum loads and stores of a lane are those
+// of an element-wise kernel (a wave touches whole lines), where variant 3
+// reads each line once per column.  In LDS, element (c, i, j) (component c,
+// line index i, column j of the block) has the row-major index
+// a = (c Nx + i) TK + j, moved within its 128-byte chunk of TB elements to
+// a ^ f(a / TB) with f(m) = (m ^ m >> s1 ^ m >> s2 ^ m >> s3) mod TB: the
+// shifts, per element size, Nx and TK, are ones under which the lanes that
+// one LDS cycle serves (the DFT groups' strided line accesses and the
+// point-wise accesses) fall in distinct banks (2 ways at most for FP32 and
+// Nx = 16, 32), found by search over all the kernel's access patterns with
+// 32 banks of 4 bytes.  Phases, separated by
+// block barriers: load -> forward line DFTs (waves as in variant 3: groups
+// of L lanes per line) -> Green operator per k point -> inverse line DFTs
+// -> store.
+// ------------------------------------------------------------------
+template <typename T, int NX, int TK> struct XC {
+    using C = typename cplx_of<T>::t;
+    static constexpr int TB = 128 / (int)sizeof(C);                 // columns per 128-byte block
+    static constexpr int L = P2Plan<NX>::L, R = P2Plan<NX>::R;
+    static constexpr int CPW = (NX == 64) ? 1 : 2;                   // columns per wave (DFT role)
+    static constexpr int WPB = TK / CPW, NT = 64 * WPB;
+    static_assert(TK == TB || 2 * TK == TB || 4 * TK == TB, "a block holds a column block, a half or a quarter");
+    static_assert(TK % CPW == 0, "whole columns per wave");
+    // the swizzle shifts (0: no term)
+    static constexpr int SW = sizeof(C) == 16
+        ? (NX == 16 ? (TK == 4 ? 0x012 : 0x002) : NX == 32 ? 0x013 : (TK == 8 ? 0x003 : TK == 4 ? 0x002 : 0x001))
+        : (NX == 16 ? (TK == 8 ? 0x001 : 0x012) : NX == 32 ? (TK == 16 ? 0x013 : TK == 8 ? 0x136 : 0x023) : (TK == 16 ? 0x003 : 0x013));
+    __device__ __forceinline__ static int shr(int m, int k) { return k ? (m >> k) : 0; }
+    __device__ __forceinline__ static int slot(int c, int i, int j)
+    {
+        const int a = (c * NX + i) * TK + j, m = a / TB;
+        return a ^ ((m ^ shr(m, SW >> 8) ^ shr(m, (SW >> 4) & 15) ^ shr(m, SW & 15)) & (TB - 1));
+    }
+};
+
+template <typename T, int NX, int TK>
+__global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *__restrict__ rtab,
+                                                               const int2 *__restrict__ bmap, int mode)
+{
+    using X = XC<T, NX, TK>;
+    using C = typename X::C;
+    constexpr int TB = X::TB, L = X::L, R = X::R, CPW = X::CPW;
+    RVE_PROLOGUE(lb);
+    const MacroState *ms = r.ms;
+    if (mode == 0 ? !iter_active(ms) : !rve_need_skew(ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int ny = r.ny, nz = r.nz, nzh = r.nzh;
+    const int ncolt = ny * nzh;
+    // the two halves of a column block run on one XCD (one L2 fetches its lines)
+    int tile = lb;
+    {
+        constexpr int G = TB / TK, span = 8 * G;
+        const int nt = (ncolt + TK - 1) / TK;
+        if (G > 1 && lb < (nt / span) * span) {
+            const int xcd = lb & 7, j = lb >> 3;
+            tile = ((j / G) * 8 + xcd) * G + (j % G);
+        }
+    }
+    const int col0 = tile * TK;                              // the block's first column
+    const long cstr = (long)r.ncb * NX * TB;
+    // element (kx, col0 + j) of a component at base + kx * TB + j
+    const long base = (long)(col0 / TB) * NX * TB + (col0 % TB);
+    // point role: thread p takes k point (kx, column j)
+    const int p = (int)threadIdx.x, pkx = p / TK, pj = p % TK;
+    const bool pt = (pkx < NX) && (col0 + pj < ncolt);
+    const long pg = base + (long)pkx * TB + pj;
+    // DFT role: group g of L lanes of wave w holds component cc of column dj
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int g = lane / L, n1 = lane % L;
+    const int cc = g % 6, dj = w * CPW + g / 6;
+    const bool dft = (g < 6 * CPW) && (col0 + dj < ncolt) && (mode == 0 || cc < 3);
+    C *spec = (C *)r.spec;
+    C *usum = (C *)r.usum;
+    const bool first = (ms->iter == 0);
+    // load
+    C u[3];
+    if (pt) {
+        if (mode == 0) {
+            C a[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a[c] = gload(spec + c * cstr + pg);
+            if (!first) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) u[i] = gload(usum + i * cstr + pg);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) s[X::slot(c, pkx, pj)] = a[c];
+        } else {
+            C a[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a[c] = gload(usum + c * cstr + pg);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s[X::slot(c, pkx, pj)] = a[c];
+        }
+    }
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_x;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
+    }
+    auto line = [&](int i) { return s + X::slot(cc, i, dj); };
+    __syncthreads();
+    if (mode == 0) {
+        if (dft) {
+            C v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = *line(n1 + L * q);
+            line_dft<C, NX, false>(v, tw, n1, line);
+            wave_lds_sync();
+#pragma unroll
+            for (int o = 0; o < R; ++o) *line(line_out_index<NX>(n1, o)) = v[o];
+        }
+        __syncthreads();
+        if (pt) {
+            const int kx = pkx, col = col0 + pj;
+            const int ky = col / nzh, kz = col - ky * nzh;
+            C a[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a[c] = s[X::slot(c, kx, pj)];
+            C d[6], wk[3];
+            if (kx == 0 && ky == 0 && kz == 0) {
+                // Gamma(k = 0) = 0 (init.c:586-591)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) d[c] = C{0, 0};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) wk[c] = C{0, 0};
+            } else {
+                const T invV = (T)(1.0 / (double)r.V);
+                T n[3];
+                kvec<T>(kx, ky, kz, NX, ny, nz, n);
+                green_apply<T, C>(green_coef<T>(ms), n, a, invV, d, wk);
+                // Hermitian average on the x and y Nyquist planes, as in k_xgreen
+                const bool nyq = (2 * kx == NX) || (2 * ky == ny);
+                if (nyq && kz > 0 && 2 * kz < nz) {
+                    T m[3];
+                    kvec<T>((NX - kx) % NX, (ny - ky) % ny, nz - kz, NX, ny, nz, m);
+                    C d2[6], w2[3];
+                    green_apply<T, C>(green_coef<T>(ms), m, a, invV, d2, w2);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) d[c] = C{(T)0.5 * (d[c].x + d2[c].x), (T)0.5 * (d[c].y + d2[c].y)};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) wk[c] = C{(T)0.5 * (wk[c].x + w2[c].x), (T)0.5 * (wk[c].y + w2[c].y)};
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) s[X::slot(c, kx, pj)] = d[c];
+            // sum of the skew parts over the increment (lattice spin)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) gstore(usum + i * cstr + pg, first ? wk[i] : cadd(u[i], wk[i]));
+        }
+        __syncthreads();
+    }
+    // inverse x DFT (mode 1: of the skew sums, into SPEC[0..2])
+    if (dft) {
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = *line(n1 + L * q);
+        line_dft<C, NX, true>(v, tw, n1, line);
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) *line(line_out_index<NX>(n1, o)) = v[o];
+    }
+    __syncthreads();
+    if (pt) {
+        if (mode == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) gstore(spec + c * cstr + pg, s[X::slot(c, pkx, pj)]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gstore(spec + c * cstr + pg, s[X::slot(c, pkx, pj)]);
+        }
     }
 }
 
@@ -1475,46 +1674,51 @@ __device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *o
     }
 }
 
-// Fixed-order sum of nb partial vectors (NV values each, stride 16) by one
-// block; the result lands in tot[] (shared).  Reads bypass L1 so that the
-// partials other blocks wrote in this launch are seen.
-// Sums rows 0..nb-1 of 16-double partial rows (the first NV entries) into
-// tot in an order that does not depend on the block size: 16 row streams,
-// stream s summing rows s, s + 16, s + 32, ... in row order, then the streams
-// added in stream order.  Lane l of a wave takes entry l % 16 of stream
-// 4 g + l / 16 of stream group g, so a load instruction reads four whole
-// 128-byte rows; four waves take one group each (fewer waves loop over the
-// groups).  Ends with a barrier.
+// Fixed-order sum of nb partial rows (16 doubles each, the first NV used) by
+// one block into tot[] (shared), in an order that does not depend on the
+// block size: S row streams, stream s summing entries e < NE of rows s,
+// s + S, s + 2 S, ... in row order, then the streams added in stream order.
+// With dup (the Newton partials of one crystalline phase: entries 8..13
+// repeat 2..7, the phase-1 stress sums being the stress sums) only entries
+// 0..7 are summed, over S = 32 streams; otherwise all 16 over S = 16.  Logical lane l = 64 w +
+// lane (256 of them, over four waves; fewer waves loop) takes entry l % NE
+// of stream l / NE, so a load instruction reads whole 128-byte rows, and a
+// lane keeps U loads in flight (one round trip per U rows of its stream).
+// Ends with a barrier.
 template <int NT, int NV>
-__device__ __forceinline__ void sum_partials(const double *part, int nb, double *tot)
+__device__ __forceinline__ void sum_partials(const double *part, int nb, bool dup, double *tot)
 {
-    constexpr int NW = NT / 64, NWU = NW < 4 ? NW : 4, PASSES = 4 / NWU, U = 16;
+    constexpr int NW = NT / 64, NWU = NW < 4 ? NW : 4, PASSES = 4 / NWU, U = 32;
     static_assert(NW == 1 || NW == 2 || NW >= 4, "waves per block");
-    __shared__ double red[16][16];
+    static_assert(NV <= 16, "at most 16 partial sums per row");
+    const bool uni = dup && NV == 14;
+    __shared__ double red[256];
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
-    const int e = lane & 15;
+    const int lne = uni ? 3 : 4, S = uni ? 32 : 16;   // NE = 1 << lne
+    const gptr<double> gp = as_global(part);
     if (w < NWU) {
 #pragma unroll 1
         for (int t = 0; t < PASSES; ++t) {
-            const int st = 4 * (w + NWU * t) + (lane >> 4);
+            const int l = 64 * (w + NWU * t) + lane;
+            const int st = l >> lne, e = l & ((1 << lne) - 1);
             double a = 0.0;
             int b = st;
-            for (; b + 16 * (U - 1) < nb; b += 16 * U) {
+            for (; b + S * (U - 1) < nb; b += S * U) {
                 double q[U];
 #pragma unroll
-                for (int k = 0; k < U; ++k) q[k] = __builtin_nontemporal_load(part + (long)(b + 16 * k) * 16 + e);
+                for (int k = 0; k < U; ++k) q[k] = __builtin_nontemporal_load(gp + (long)(b + S * k) * 16 + e);
 #pragma unroll
                 for (int k = 0; k < U; ++k) a += q[k];
             }
-            for (; b < nb; b += 16) a += __builtin_nontemporal_load(part + (long)b * 16 + e);
-            red[st][e] = a;
+            for (; b < nb; b += S) a += __builtin_nontemporal_load(gp + (long)b * 16 + e);
+            red[l] = a;
         }
     }
     __syncthreads();
     if (threadIdx.x < NV) {
+        const int e = (uni && threadIdx.x >= 8) ? (int)threadIdx.x - 6 : (int)threadIdx.x;
         double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) acc += red[k][threadIdx.x];
+        for (int k = 0; k < S; ++k) acc += red[(k << lne) + e];
         tot[threadIdx.x] = acc;
     }
     __syncthreads();
@@ -2008,13 +2212,13 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
     if (v < V) {
-        const uint32_t ori = r.ORI[v];
-        const int ph = r.PH[v];
+        const uint32_t ori = as_global(r.ORI)[v];
+        const int ph = as_global(r.PH)[v];
         if (ori == VOCE_GAS) {
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                r.S6[i * V + v] = 0.0;
-                r.P6[i * V + v] = 0.0;
+                as_global_mut(r.S6)[i * V + v] = 0.0;
+                as_global_mut(r.P6)[i * V + v] = 0.0;
             }
         } else {
             const cptr<PhaseConst> pc = as_const(r.phase + (UNI ? 0 : (ph - 1)));
@@ -2023,12 +2227,23 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             const double dt = msc->dt;
             const gptr<double> tab = as_global(r.SLOC + (long)ori * VOCE_OTAB);   // S_local, K_c, initial TranMat
             // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
-            // it would form per evaluation)
+            // it would form per evaluation); the arrays hold twelve systems
+            // per voxel, so all twelve loads go out together
+            {
+                double cq[12];
+                const gptr<double> cf = as_global(r.CRSS + v);
 #pragma unroll
-            for (int q = 0; q < 12; ++q) {
-                if (q < nsys) {
-                    lds_ic[q][threadIdx.x] = 1.0 / r.CRSS[q * V + v];
-                    if (TWO) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / r.CRSSB[q * V + v];
+                for (int q = 0; q < 12; ++q) cq[q] = cf[(long)q * V];
+#pragma unroll
+                for (int q = 0; q < 12; ++q)
+                    if (q < nsys) lds_ic[q][threadIdx.x] = 1.0 / cq[q];
+                if (TWO) {
+                    const gptr<double> cb = as_global(r.CRSSB + v);
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) cq[q] = cb[(long)q * V];
+#pragma unroll
+                    for (int q = 0; q < 12; ++q)
+                        if (q < nsys) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / cq[q];
                 }
             }
             // The Newton loop runs in the crystal frame of TranMat R, which is
@@ -2043,7 +2258,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             {
                 double Rm[9];
 #pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+                for (int i = 0; i < 9; ++i) Rm[i] = as_global(r.R)[i * V + v];
                 if (msc->iter == 0) {
                     // first iteration of the increment: K_c in the frame of
                     // this increment's TranMat
@@ -2054,8 +2269,8 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                 double s0[6], w[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    s0[i] = r.S6[i * V + v];
-                    w[i] = (i < 5 ? r.EP5[i * V + v] : 0.0) - r.E6[i * V + v];
+                    s0[i] = as_global(r.S6)[i * V + v];
+                    w[i] = (i < 5 ? as_global(r.EP5)[i * V + v] : 0.0) - as_global(r.E6)[i * V + v];
                     signorm += s0[i] * s0[i];
                 }
                 // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
@@ -2107,7 +2322,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             {
                 double Rm[9];
 #pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
+                for (int i = 0; i < 9; ++i) Rm[i] = as_global(r.R)[i * V + v];
                 double sx[5], ox[5], ex[5], wx[5];
 #pragma unroll
                 for (int i = 0; i < 5; ++i) {
@@ -2142,10 +2357,10 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             double lam[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                lam[i] = r.S6[i * V + v];
+                lam[i] = as_global(r.S6)[i * V + v];
                 cs += (sig[i] - lam[i]) * (sig[i] - lam[i]);
-                r.S6[i * V + v] = sig[i];
-                r.P6[i * V + v] = sig_old[i];
+                as_global_mut(r.S6)[i * V + v] = sig[i];
+                as_global_mut(r.P6)[i * V + v] = sig_old[i];
             }
             {
                 const cptr<double> S0b = msc->S066;
@@ -2205,7 +2420,7 @@ __device__ __forceinline__ void finalize_block(const RveDev &r)
     }
     __shared__ double tot[16];
     __shared__ double dDs[9];
-    sum_partials<NT, 14>(r.part, r.nblk_vox, tot);   // ends with a barrier
+    sum_partials<NT, 14>(r.part, r.nblk_vox, r.uniform_phase != 0, tot);   // ends with a barrier
     if (threadIdx.x == 0) {
         finalize_iteration(ms, &sms, tot, dDs);
         ms->fin_pending = 0;
@@ -2530,7 +2745,7 @@ __global__ __launch_bounds__(NT) void k_fin_incr(const RveDev *__restrict__ rtab
     (void)lb;
     MacroState *ms = r.ms;
     __shared__ double tot[16];
-    sum_partials<NT, 10>(r.part, r.nblk_vox, tot);
+    sum_partials<NT, 10>(r.part, r.nblk_vox, false, tot);
     if (threadIdx.x != 0) return;
     const double dt = ms->dt;
     for (int i = 0; i < 9; ++i) {
@@ -2696,6 +2911,20 @@ void launch_xgreen_p2(const LaunchSet *L, hipStream_t st, int mode)
     else VLAUNCH((k_xgreen_p2<T, NX, 2>), dim3(L->n_x), dim3(128), L->x_lds, st, L->tab, L->map_x, mode);
 }
 
+template <typename T, int NX>
+void launch_xgreen_c(const LaunchSet *L, hipStream_t st, int mode)
+{
+    constexpr int TB = 128 / (int)sizeof(typename cplx_of<T>::t);
+    if (L->p2_tk == TB)
+        VLAUNCH((k_xgreen_c<T, NX, TB>), dim3(L->n_x), dim3(XC<T, NX, TB>::NT), L->x_lds, st, L->tab, L->map_x, mode);
+    else if (L->p2_tk == TB / 2)
+        VLAUNCH((k_xgreen_c<T, NX, TB / 2>), dim3(L->n_x), dim3(XC<T, NX, TB / 2>::NT), L->x_lds, st, L->tab, L->map_x,
+                mode);
+    else
+        VLAUNCH((k_xgreen_c<T, NX, TB / 4>), dim3(L->n_x), dim3(XC<T, NX, TB / 4>::NT), L->x_lds, st, L->tab, L->map_x,
+                mode);
+}
+
 template <typename T>
 void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
 {
@@ -2704,6 +2933,11 @@ void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
         if (L->p2_nx == 16) launch_xgreen_p2<T, 16>(L, st, mode);
         else if (L->p2_nx == 32) launch_xgreen_p2<T, 32>(L, st, mode);
         else launch_xgreen_p2<T, 64>(L, st, mode);
+        break;
+    case 4:   // x pass staged through LDS (the grid's nx)
+        if (L->p2_nx == 16) launch_xgreen_c<T, 16>(L, st, mode);
+        else if (L->p2_nx == 32) launch_xgreen_c<T, 32>(L, st, mode);
+        else launch_xgreen_c<T, 64>(L, st, mode);
         break;
     case 0: VLAUNCH((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
     case 1: VLAUNCH((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
@@ -2765,7 +2999,13 @@ hipError_t set_lds_limits()
                          (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>,
                          (const void *)k_xgreen_p2<T, 16, 2>, (const void *)k_xgreen_p2<T, 16, 4>,
                          (const void *)k_xgreen_p2<T, 32, 2>, (const void *)k_xgreen_p2<T, 32, 4>,
-                         (const void *)k_xgreen_p2<T, 64, 2>, (const void *)k_xgreen_p2<T, 64, 4>};
+                         (const void *)k_xgreen_p2<T, 64, 2>, (const void *)k_xgreen_p2<T, 64, 4>,
+                         (const void *)k_xgreen_c<T, 16, XC<T, 16, 8>::TB>, (const void *)k_xgreen_c<T, 16, XC<T, 16, 8>::TB / 2>,
+                         (const void *)k_xgreen_c<T, 16, XC<T, 16, 8>::TB / 4>,
+                         (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB>, (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 2>,
+                         (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 4>,
+                         (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB>, (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 2>,
+                         (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 4>};
     for (const void *f : fns) {
         // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
         hipFuncAttributes fa;
@@ -2809,6 +3049,7 @@ static const size_t LDS_MAX = 160 * 1024;
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
     if (xv == 3) return (nx == 16 || nx == 32) ? 4 : (nx == 64 ? 2 : 0);
+    if (xv == 4) return (nx == 16 || nx == 32 || nx == 64) ? 8 : 0;   // FP64: a whole column block
     const int fit = X_MAXEL[xv] / (6 * nx);
     if (fit < 1) return 0;   // the variant cannot hold one column
     int tk = 1;
@@ -2849,6 +3090,15 @@ extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
         const int nx = r->nx, cpw = nx == 64 ? 1 : 2;
         if (nx != 16 && nx != 32 && nx != 64) return 0;
         if (tk != 2 * cpw && tk != 4 * cpw) return 0;
+        return (size_t)tk * 6 * nx * cb;
+    }
+    if (xv == 4) {
+        // x pass staged through LDS: Nx 16, 32 or 64; tk = a whole column
+        // block (8 FP64 / 16 FP32 columns), a half or a quarter of one (whole
+        // columns per wave: 1 at Nx = 64, else 2); six lines of Nx per column
+        const int nx = r->nx, tb = r->fft_f32 ? 16 : 8, cpw = nx == 64 ? 1 : 2;
+        if (nx != 16 && nx != 32 && nx != 64) return 0;
+        if ((tk != tb && 2 * tk != tb && 4 * tk != tb) || tk % cpw) return 0;
         return (size_t)tk * 6 * nx * cb;
     }
     if (xv < 0 || xv > 2 || tk < 1 || (tk & (tk - 1)) || 6 * tk * r->nx > X_MAXEL[xv]) return 0;

@@ -141,7 +141,7 @@ int measure(voce_ctx *c, voce_tune_choice *best)
     const int tks[5] = {1, 2, 4, 8, 16};
     for (int xv = 0; xv < VOCE_N_XV && !rc; ++xv)
         for (int ti = 0; ti < 5 && !rc; ++ti)
-            for (int r8 = 1; r8 >= (xv == 3 ? 1 : 0) && !rc; --r8) {   // xv 3 has fixed radices
+            for (int r8 = 1; r8 >= (xv >= 3 ? 1 : 0) && !rc; --r8) {   // xv 3 and 4 have fixed radices
                 cand = keep;
                 cand.xv = xv;
                 cand.tk = tks[ti];
