@@ -385,16 +385,22 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
 
 template <int NT> __device__ __forceinline__ void finalize_block(const RveDev &r);
 
+// The macro state is written only by single-thread finalisers in other
+// launches (or, racing harmlessly, by the finaliser block of a forward plane
+// launch, whose other blocks only write scratch spectra), so kernels read it
+// through the scalar cache: a vector load would be waited for together with
+// every load issued before it.
 __device__ __forceinline__ bool iter_active(const MacroState *ms)
 {
-    return ms->active != 0;
+    return as_const(ms)->active != 0;
 }
 
 // The increment's spin is needed by update_orient from step 2
 // (evolution.c:500), by the creep predictor and by the diagnostics.
 __device__ __forceinline__ bool rve_need_skew(const MacroState *ms)
 {
-    return (ms->update_flag == 1 && ms->istep > 1) || ms->creep || ms->diag;
+    const cptr<MacroState> m = as_const(ms);
+    return (m->update_flag == 1 && m->istep > 1) || m->creep || m->diag;
 }
 
 // ------------------------------------------------------------------
@@ -549,7 +555,7 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
     const T *sr = (const T *)s;
     const long off = (long)c * r.V + (long)x * ny * nz;
     if (mode == 0) {
-        const MacroState *ms = r.ms;
+        const cptr<MacroState> ms = as_const(r.ms);
         const gmut<double> E = as_global_mut(r.E6 + off);
         const bool creep_set = (ms->creep != 0) && (ms->iter == 0);
         const double sh = creep_set ? 0.0 : ms->shift6[c];
@@ -755,21 +761,9 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
     const int nx = r.nx;
     const int c = lb / nx, x = lb - c * nx;
     const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
-    const MacroState *ms = r.ms;
+    const cptr<MacroState> ms = as_const(r.ms);
     const bool creep_set = mode == 0 && (ms->creep != 0) && (ms->iter == 0);
     const long off = (long)c * r.V + ((long)x * N + 2 * grp) * N;   // row 2p of this group
-    // E of the two rows this lane updates, loaded now so that the loads
-    // are in flight through both halves of the pass
-    // (pairs of points, consecutive lanes on consecutive pairs)
-    double2 ev[2][R / 2];
-    if (mode == 0 && !creep_set && grp < N / 2) {
-        const double2 *E = (const double2 *)(r.E6 + off);
-#pragma unroll
-        for (int q = 0; q < R / 2; ++q) {
-            ev[0][q] = gload(E + n1 + L * q);
-            ev[1][q] = gload(E + N / 2 + n1 + L * q);
-        }
-    }
     C tw[R];
     {
         const C *gtw = (const C *)r.tw_z;
@@ -791,6 +785,19 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
         wave_lds_sync();
 #pragma unroll
         for (int o = 0; o < R; ++o) col[line_out_index<N>(n1, o) * NZH] = v[o];
+    }
+    // E of the two rows this lane updates (pairs of points, consecutive
+    // lanes on consecutive pairs), loaded once the spectrum is in: issued at
+    // the start, these loads made the y pass wait for them as well (a wait
+    // on a load also waits on every load issued before it)
+    double2 ev[2][R / 2];
+    if (mode == 0 && !creep_set && grp < N / 2) {
+        const double2 *E = (const double2 *)(r.E6 + off);
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            ev[0][q] = gload(E + n1 + L * q);
+            ev[1][q] = gload(E + N / 2 + n1 + L * q);
+        }
     }
     __syncthreads();
     // z: rows 2p (a) and 2p+1 (b) as one complex row Z = A + iB, A and B
@@ -1052,7 +1059,7 @@ __global__ __launch_bounds__(NT) void k_xgreen(const RveDev *__restrict__ rtab, 
     }
     const int col0 = tile * tk;
     const int ncol = min(tk, ncolt - col0);
-    const bool first = (ms->iter == 0);
+    const bool first = (as_const(ms)->iter == 0);
     const C *src = (const C *)((mode == 0) ? r.spec : r.usum);
     C *usum = (C *)r.usum;
     const int lgt = 31 - __builtin_clz(tk);   // tk is a power of two
@@ -1176,7 +1183,7 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
     // Green role: lane p takes k point (column j2, kx)
     const int j2 = lane / NX, kx = lane % NX, col2 = wc0 + j2;
     const bool pt = (lane < CPW * NX) && (col2 < ncolt) && mode == 0;
-    const bool first = (ms->iter == 0);
+    const bool first = (as_const(ms)->iter == 0);
     C *usum = (C *)r.usum;
     C u[3];
     if (pt && !first) {
@@ -1340,7 +1347,15 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     const bool dft = (g < 6 * CPW) && (col0 + dj < ncolt) && (mode == 0 || cc < 3);
     C *spec = (C *)r.spec;
     C *usum = (C *)r.usum;
-    const bool first = (ms->iter == 0);
+    const bool first = (as_const(ms)->iter == 0);
+    // the twiddles first: a wait on a load also waits on every load issued
+    // before it, so the DFTs then wait for the spectra only
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_x;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
+    }
     // load
     C u[3];
     if (pt) {
@@ -1348,10 +1363,13 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
             C a[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) a[c] = gload(spec + c * cstr + pg);
-            if (!first) {
+            // the running skew sums after the spectra (kept in that order: a
+            // wait on a load also waits on the loads issued before it), and
+            // on the first iteration too (then not used), since a load under
+            // a branch makes the waits after it count from the shorter path
+            __asm__ volatile("" ::: "memory");
 #pragma unroll
-                for (int i = 0; i < 3; ++i) u[i] = gload(usum + i * cstr + pg);
-            }
+            for (int i = 0; i < 3; ++i) u[i] = gload(usum + i * cstr + pg);
 #pragma unroll
             for (int c = 0; c < 6; ++c) s[X::slot(c, pkx, pj)] = a[c];
         } else {
@@ -1360,13 +1378,10 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
             for (int c = 0; c < 3; ++c) a[c] = gload(usum + c * cstr + pg);
 #pragma unroll
             for (int c = 0; c < 3; ++c) s[X::slot(c, pkx, pj)] = a[c];
+            // keeps this tail apart from the other branch's (a shared tail
+            // would take this path's shorter wait counts)
+            __asm__ volatile("" ::: "memory");
         }
-    }
-    C tw[R];
-    {
-        const C *gtw = (const C *)r.tw_x;
-#pragma unroll
-        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
     }
     auto line = [&](int i) { return s + X::slot(cc, i, dj); };
     __syncthreads();
@@ -2226,25 +2241,34 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             const cptr<MacroState> msc = as_const(ms);
             const double dt = msc->dt;
             const gptr<double> tab = as_global(r.SLOC + (long)ori * VOCE_OTAB);   // S_local, K_c, initial TranMat
-            // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c
-            // it would form per evaluation); the arrays hold twelve systems
-            // per voxel, so all twelve loads go out together
+            // Every load of the prologue goes out first, before any branch
+            // or use: a wait on a load also waits on the loads issued before
+            // it, and a load behind a branch or a use costs a round trip of
+            // its own.  The CRSS arrays hold twelve systems per voxel.
+            double cq[12], Rm[9], s0[6], w[6];
             {
-                double cq[12];
                 const gptr<double> cf = as_global(r.CRSS + v);
 #pragma unroll
                 for (int q = 0; q < 12; ++q) cq[q] = cf[(long)q * V];
 #pragma unroll
-                for (int q = 0; q < 12; ++q)
-                    if (q < nsys) lds_ic[q][threadIdx.x] = 1.0 / cq[q];
-                if (TWO) {
-                    const gptr<double> cb = as_global(r.CRSSB + v);
+                for (int i = 0; i < 9; ++i) Rm[i] = as_global(r.R)[i * V + v];
 #pragma unroll
-                    for (int q = 0; q < 12; ++q) cq[q] = cb[(long)q * V];
-#pragma unroll
-                    for (int q = 0; q < 12; ++q)
-                        if (q < nsys) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / cq[q];
+                for (int i = 0; i < 6; ++i) {
+                    s0[i] = as_global(r.S6)[i * V + v];
+                    w[i] = (i < 5 ? as_global(r.EP5)[i * V + v] : 0.0) - as_global(r.E6)[i * V + v];
                 }
+            }
+            // the flow rule only needs 1/CRSS (the same quotient 1.0/tau_c it
+            // would form per evaluation); systems past nsys (CRSS 0) are
+            // never read
+#pragma unroll
+            for (int q = 0; q < 12; ++q) lds_ic[q][threadIdx.x] = 1.0 / cq[q];
+            if (TWO) {
+                const gptr<double> cb = as_global(r.CRSSB + v);
+#pragma unroll
+                for (int q = 0; q < 12; ++q) cq[q] = cb[(long)q * V];
+#pragma unroll
+                for (int q = 0; q < 12; ++q) lds_ic[TWO ? 12 + q : q][threadIdx.x] = 1.0 / cq[q];
             }
             // The Newton loop runs in the crystal frame of TranMat R, which is
             // fixed during an increment: with Q the B-basis rotation of R and
@@ -2256,9 +2280,6 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             // never touches R.
             double sig[6], signorm = 0.0;
             {
-                double Rm[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Rm[i] = as_global(r.R)[i * V + v];
                 if (msc->iter == 0) {
                     // first iteration of the increment: K_c in the frame of
                     // this increment's TranMat
@@ -2266,13 +2287,8 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                     basis_rotation(Rm, Q);
                     store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
                 }
-                double s0[6], w[6];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    s0[i] = as_global(r.S6)[i * V + v];
-                    w[i] = (i < 5 ? as_global(r.EP5)[i * V + v] : 0.0) - as_global(r.E6)[i * V + v];
-                    signorm += s0[i] * s0[i];
-                }
+                for (int i = 0; i < 6; ++i) signorm += s0[i] * s0[i];
                 // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
                 // part of the compliance-form residual below, kept in LDS (a
                 // per-lane register extension)
