@@ -166,3 +166,32 @@ def test_batch_radix5_radix7_odd_members(tmp_path):
             assert rb[k][i]["iter"] == ro[k]["iter"]
             assert rel(rb[k][i]["sig_avg"], ro[k]["sig_avg"]) < 1e-8
         assert rel(fb[i]["sig"], fo["sig"]) < 1e-7
+
+
+def test_batch_member_with_split_plane_passes(tmp_path):
+    """A 6 x 256 x 8 member (planes past one block's LDS: the split z and y
+    passes) batched with a 16^3 one: each equals its stand-alone run bit for
+    bit and matches the oracle."""
+    specs = [inputs.RveSpec(d, 4, 1300 + i, np.zeros((3, 3))) for i, d in enumerate([(16, 16, 16), (6, 256, 8)])]
+    for sp in specs:
+        sp.velgrad = np.diag([1e-3, -0.5e-3, -0.5e-3])
+    cases = inputs.load_specs(str(tmp_path / "b"), specs)
+    n = 2
+    ra, fa = _run_alone(cases, n, "fp64")
+    rb, fb, info = _run_batch(cases, n, "fp64", tune=False)
+    for i, sp in enumerate(specs):
+        for k in range(n):
+            assert ra[i][k]["iter"] == rb[k][i]["iter"]
+            assert np.array_equal(ra[i][k]["sig_avg"], rb[k][i]["sig_avg"])
+        for f in FIELDS:
+            assert np.array_equal(fa[i][f], fb[i][f]), (i, f)
+        c = inputs.full_velocity_case(dims=sp.dims)
+        c.n_grains, c.seed = sp.n_grains, sp.seed
+        c.velgrad = (1e-3, -0.5e-3, -0.5e-3, 0.0, 0.0, 0.0)
+        path, base = write(tmp_path / ("o%d" % i), c)
+        ro, fo = oracle_run(path, base, n, fields=("sig",))
+        for k in range(n):
+            say(sp.dims, "incr", k + 1, "iters batch/oracle", rb[k][i]["iter"], ro[k]["iter"])
+            assert rb[k][i]["iter"] == ro[k]["iter"]
+            assert rel(rb[k][i]["sig_avg"], ro[k]["sig_avg"]) < 1e-8
+        assert rel(fb[i]["sig"], fo["sig"]) < 1e-7

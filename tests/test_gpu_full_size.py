@@ -72,3 +72,43 @@ def test_config3_128cube_mixed_against_fp64(tmp_path):
     fr = rel(out["mixed_sig"], out["fp64_sig"])
     say("per-voxel stress mixed vs fp64 (max abs / max)", fr)
     assert fr < 1e-3
+
+
+def test_256cube_split_planes_mixed_against_fp64(tmp_path):
+    """A 256^3 RVE (16.8 M voxels, 1000 grains, plane-strain full velocity
+    control as in the 128^3 case): its planes are past one block's LDS, so
+    the split z and y passes run, and the x pass holds 256-point columns.
+    Too large for the oracle in a test; checked by MIXED against FP64 (1e-4
+    macro) and the size-independent properties the load fixes."""
+    c = inputs.config_case(3)
+    d = str(tmp_path)
+    c.iter_max, c.n_steps = 20, 2
+    inputs.write_case(d, inputs.Case(dims=(8, 8, 8), velgrad_flag=c.velgrad_flag, velgrad=c.velgrad,
+                                     stress_flag=c.stress_flag, time_step=c.time_step, err=c.err,
+                                     iter_max=c.iter_max, n_steps=c.n_steps))
+    cs = voce_amd.load_case_voronoi(os.path.join(d, "input"), d, (256, 256, 256), 1000, 3)
+    n = 2
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = voce_amd.Solver.from_case(cs, prec)
+        t = s.tuning()
+        say(prec, "launch choice", t)
+        assert t["plv"] == 5
+        out[prec] = [s.increment() for _ in range(n)]
+        out[prec + "_sig"] = s.field("sig")
+        s.close()
+        say(prec, [r["iter"] for r in out[prec]])
+    L = np.zeros((3, 3))
+    L[0, 0], L[2, 2] = c.velgrad[0], c.velgrad[2]
+    for k in range(n):
+        a, b = out["mixed"][k], out["fp64"][k]
+        r = rel(a["sig_avg"], b["sig_avg"])
+        say("increment", k + 1, "macro stress mixed vs fp64", r)
+        assert r < 1e-4
+        assert a["iter"] == b["iter"] == c.iter_max
+        for rep in (a, b):
+            assert np.abs(rep["velgrad_avg"] - L).max() < 1e-15
+            assert np.abs(rep["sig_avg"] - rep["sig_avg"].T).max() <= 1e-12 * np.abs(rep["sig_avg"]).max()
+    fr = rel(out["mixed_sig"], out["fp64_sig"])
+    say("per-voxel stress mixed vs fp64 (max abs / max)", fr)
+    assert fr < 1e-3

@@ -237,6 +237,36 @@ def test_fp64_parity_radix5_radix7_odd_planes(tmp_path, dims, n):
     compare(tmp_path, c, n)
 
 
+@pytest.mark.parametrize("dims", [(16, 16, 16), (12, 10, 14), (9, 7, 8), (8, 30, 20)])
+def test_fp64_parity_split_plane_passes_forced(tmp_path, dims):
+    # plane variant 5 (separate z and y launches, each block walking its
+    # plane in chunks) on grids the whole-plane kernels also run, against
+    # the oracle; 7 is an odd Ny (its last row paired with zeros)
+    c = inputs.Case(dims=dims, n_grains=5, seed=31, n_steps=2)
+    compare(tmp_path, c, 2, tuning={"plv": 5})
+
+
+@pytest.mark.parametrize("dims", [(4, 8, 200), (6, 256, 8), (256, 6, 4), (5, 144, 150)])
+def test_fp64_parity_dimensions_above_128(tmp_path, dims):
+    # a dimension beyond 128: planes past one block's LDS take the split
+    # passes (the default choice there), Nx = 256 the LDS-tiled x pass with a
+    # 2-column tile; 200 = 8 x 5^2, 144 = 16 x 9, 150 = 2 x 3 x 5^2
+    c = inputs.Case(dims=dims, n_grains=5, seed=37, n_steps=2)
+    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "new_position"))
+
+
+def test_mixed_split_plane_passes_against_oracle(tmp_path):
+    c = inputs.Case(dims=(6, 256, 8), n_grains=5, seed=41, n_steps=2)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",))
+    o, of = oracle_run(path, base, 2, fields=("sig",))
+    for k in range(2):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed (split plane passes) vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-5
+
+
 def test_fp64_parity_9cube_elastic_converges(tmp_path):
     # all-odd grid: no Nyquist planes, the elastic iteration converges early
     c = inputs.Case(dims=(9, 9, 9), n_grains=4, seed=3, n_steps=2, material=inputs.Material(tau0f=1e4, tau0b=1e4))

@@ -326,7 +326,7 @@ extern "C" const char *voce_strerror(int code)
     case VOCE_ERR_MESH_INDEX: return "inconsistent index in the microstructure file (reference abort 1139)";
     case VOCE_ERR_ARG: return "invalid argument";
     case VOCE_ERR_FILE: return "cannot open or parse an input file";
-    case VOCE_ERR_UNSUPPORTED_GRID: return "grid size not supported (each dimension 2^a 3^b 5^c 7^d, 2..128)";
+    case VOCE_ERR_UNSUPPORTED_GRID: return "grid size not supported (each dimension 2^a 3^b 5^c 7^d, 2..512)";
     case VOCE_ERR_DEVICE: return "HIP runtime error";
     case VOCE_ERR_NOMEM: return "device memory allocation failed";
     case VOCE_ERR_NOT_LOADED: return "HIP library not loaded";
@@ -705,8 +705,7 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     d.uniform_phase = (!pcs[0].gas) && (su->n_phases == 1 || pcs[1].gas);
     d.two_crss = two_any;
     // default FFT launch choices (a tuned set replaces them)
-    d.plv = voce_plane_lds(&d, 1) > 0 ? 1 : 2;
-    if (voce_plane_lds(&d, 4) > 0) d.plv = 4;   // line-parallel planes when the grid allows them
+    d.plv = voce_default_plv(&d);   // line-parallel planes when the grid allows them
     d.xv = 0;
     d.tk = voce_xgreen_tile(nx, 0);
     d.pl_r8 = d.x_r8 = 1;

@@ -170,7 +170,8 @@ int voce_launch_finalize(const LaunchSet *L, void *stream);   // pending iterati
 int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_dev, void *stream);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
-#define VOCE_N_PLV 5                // plane-kernel variants (4: line-parallel square power-of-two planes)
+#define VOCE_N_PLV 6                // plane-kernel variants (4: line-parallel square power-of-two planes,
+                                    // 5: split z and y passes)
 // nonzero B-basis components (bit a) of the crystal-frame Schmid vectors of
 // the twelve {111}<110> systems in the order of the reference's Cu_p.sx
 // (three systems per plane)
@@ -180,6 +181,7 @@ int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);
 int voce_vox_block(void);   // threads per block of the per-voxel kernels
 size_t voce_plane_lds(const RveDev *r, int plv);            // 0: does not fit
+int voce_default_plv(const RveDev *d);                      // -1: no plane variant fits
 size_t voce_xgreen_lds(const RveDev *r, int xv, int tk);    // 0: does not fit
 void voce_spec_layout(RveDev *r);   // sets ncb and nspec from the grid and precision
 #ifdef __cplusplus

@@ -591,6 +591,226 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
 }
 
 // ------------------------------------------------------------------
+// Split plane passes (plane variant 5), for planes too large for one
+// block's LDS (any Ny, Nz up to 512).  The z and y transforms of a plane are
+// two launches; a block owns one (component, x-plane) as in the whole-plane
+// kernels (so the block maps are the same) and walks it in chunks of lines
+// that fit MAXEL_SP elements of LDS.  Between the two launches the half
+// spectrum of z row y sits in the spectrum array at column y * nzh + kz (the
+// column-blocked layout of the final spectrum, with y in place of ky); the
+// y pass transforms each kz column in place.  An odd Ny pairs its last row
+// with a row of zeros, as in the whole-plane kernels.
+// ------------------------------------------------------------------
+constexpr int NT_SP = 256, MAXEL_SP = 4096;
+constexpr int SP_MAXN = 512;   // largest Ny, Nz of the split passes
+
+template <typename C>
+__device__ __forceinline__ long spec_at(int col, int x, int nx)
+{
+    constexpr int TB = 128 / (int)sizeof(C);
+    return ((long)(col / TB) * nx + x) * TB + (col % TB);
+}
+
+// forward z pass: S6 rows (packed in pairs) -> half spectra; the extra block
+// lb == 6 nx finalises the previous iteration
+template <typename T>
+__global__ __launch_bounds__(NT_SP) void k_zfwd_s(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
+{
+    using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
+    if (lb == 6 * r.nx) {
+        finalize_block<NT_SP>(r);
+        return;
+    }
+    if (!iter_active(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh, tid = threadIdx.x;
+    const int RS = 2 * nzh, np = (ny + 1) >> 1;
+    C *twz = s + MAXEL_SP;
+    for (int i = tid; i < nz; i += NT_SP) twz[i] = gload((const C *)r.tw_z + i);
+    const int c = lb / nx, x = lb - c * nx;
+    const gptr<double> src = as_global(r.S6 + (long)c * r.V + (long)x * ny * nz);
+    C *dst = (C *)r.spec + (long)c * r.nspec;
+    T *sr = (T *)s;
+    const FastDiv bynz(nz), bynzh(nzh);
+    const int ppc = MAXEL_SP / RS;   // row pairs per chunk
+    for (int p0 = 0; p0 < np; p0 += ppc) {
+        const int npc = min(ppc, np - p0);
+        const int nrow = min(2 * npc, ny - 2 * p0);
+        grouped<NT_SP, 16, double>(nrow * nz, [&](int e) { return src[(long)2 * p0 * nz + e]; },
+                                   [&](int e, double v) {
+                                       int z;
+                                       const int y = bynz.div(e, z);
+                                       sr[2 * ((y >> 1) * RS + z) + (y & 1)] = (T)v;
+                                   });
+        if (nrow < 2 * npc)
+            for (int z = tid; z < nz; z += NT_SP) sr[2 * ((npc - 1) * RS + z) + 1] = (T)0;
+        __syncthreads();
+        fft_lines<C, NT_SP, MAXEL_SP, false, false>(s, npc, nz, RS, 1, twz, r.pl_r8);
+        // split packed rows a + i b into the half spectra of a and b, out to
+        // columns y nzh + k
+        for (int it = tid; it < npc * nzh; it += NT_SP) {
+            int k;
+            const int p = bynzh.div(it, k);
+            const C z0 = s[p * RS + k];
+            const C z1 = s[p * RS + ((k == 0) ? 0 : nz - k)];
+            const int y = 2 * (p0 + p);
+            gstore(dst + spec_at<C>(y * nzh + k, x, nx), C{(T)0.5 * (z0.x + z1.x), (T)0.5 * (z0.y - z1.y)});
+            if (y + 1 < ny)
+                gstore(dst + spec_at<C>((y + 1) * nzh + k, x, nx), C{(T)0.5 * (z0.y + z1.y), (T)0.5 * (z1.x - z0.x)});
+        }
+        __syncthreads();
+    }
+}
+
+// y pass, forward or inverse, in place on the kz columns of one plane
+template <typename T, bool INV>
+__global__ __launch_bounds__(NT_SP) void k_y_s(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nzh = r.nzh, tid = threadIdx.x;
+    C *twy = s + MAXEL_SP;
+    for (int i = tid; i < ny; i += NT_SP) twy[i] = gload((const C *)r.tw_y + i);
+    const int c = lb / nx, x = lb - c * nx;
+    C *sp = (C *)r.spec + (long)c * r.nspec;
+    const int kpc = MAXEL_SP / ny;   // kz columns per chunk
+    for (int k0 = 0; k0 < nzh; k0 += kpc) {
+        const int nk = min(kpc, nzh - k0);
+        const FastDiv bynk(nk);
+        // element e = y nk + j: consecutive lanes on consecutive columns
+        grouped<NT_SP, 8, C>(ny * nk, [&](int e) {
+            int j;
+            const int y = bynk.div(e, j);
+            return gload(sp + spec_at<C>(y * nzh + k0 + j, x, nx));
+        }, [&](int e, C v) {
+            int j;
+            const int y = bynk.div(e, j);
+            s[j * ny + y] = v;
+        });
+        __syncthreads();
+        fft_lines<C, NT_SP, MAXEL_SP, INV, false>(s, nk, ny, ny, 1, twy, r.pl_r8);
+        for (int e = tid; e < ny * nk; e += NT_SP) {
+            int j;
+            const int y = bynk.div(e, j);
+            gstore(sp + spec_at<C>(y * nzh + k0 + j, x, nx), s[j * ny + y]);
+        }
+        __syncthreads();
+    }
+}
+
+// inverse z pass: half spectra of row pairs -> real rows; mode 0: E6[c] +=
+// shift[c] + result (creep: E6[c] = creep[c] + result); mode 1: WF[c] =
+// result
+template <typename T>
+__global__ __launch_bounds__(NT_SP) void k_zinv_s(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    RVE_PROLOGUE(lb);
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx, ny = r.ny, nz = r.nz, nzh = r.nzh, tid = threadIdx.x;
+    const int RS = 2 * nzh, np = (ny + 1) >> 1;
+    C *twz = s + MAXEL_SP;
+    for (int i = tid; i < nz; i += NT_SP) twz[i] = gload((const C *)r.tw_z + i);
+    const int c = lb / nx, x = lb - c * nx;
+    const C *sp = (const C *)r.spec + (long)c * r.nspec;
+    const FastDiv bynz(nz), bynzh(nzh);
+    const cptr<MacroState> ms = as_const(r.ms);
+    const bool creep_set = mode == 0 && (ms->creep != 0) && (ms->iter == 0);
+    const double sh = (mode == 0 && !creep_set) ? ms->shift6[c] : 0.0;
+    const double cr = mode == 0 ? ms->creep_sym6[c] : 0.0;
+    const long off = (long)c * r.V + (long)x * ny * nz;
+    const int fs = (c == 1) ? 0 : (c == 2) ? 1 : (c == 3) ? 2 : (c == 5) ? 3 : -1;
+    double *F = nullptr;
+    if (r.FLU && mode == 0 && fs >= 0) F = r.FLU + (long)fs * r.V + (long)x * ny * nz;
+    if (r.FLU && mode == 1 && c < 2) F = r.FLU + (long)(4 + c) * r.V + (long)x * ny * nz;
+    const T *sr = (const T *)s;
+    const int ppc = MAXEL_SP / RS;
+    for (int p0 = 0; p0 < np; p0 += ppc) {
+        const int npc = min(ppc, np - p0);
+        const int nrow = min(2 * npc, ny - 2 * p0);
+        // the chunk's half-spectrum rows, row y - 2 p0 at (y - 2 p0) nzh
+        grouped<NT_SP, 8, C>(nrow * nzh, [&](int e) {
+            int k;
+            const int yl = bynzh.div(e, k);
+            return gload(sp + spec_at<C>((2 * p0 + yl) * nzh + k, x, nx));
+        }, [&](int e, C v) { s[e] = v; });
+        __syncthreads();
+        // pack rows 2p (a) and 2p+1 (b) into Z = A + iB, A and B
+        // Hermitian-extended (an odd last row pairs with zeros)
+        constexpr int SPT = (MAXEL_SP + NT_SP - 1) / NT_SP;
+        C Z[SPT];
+        const int nit = npc * nz;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT_SP;
+            if (it < nit) {
+                int k;
+                const int p = bynz.div(it, k);
+                const bool hb = 2 * p + 1 < nrow;
+                C A, B;
+                if (k <= nz / 2) {
+                    A = s[(2 * p) * nzh + k];
+                    B = hb ? s[(2 * p + 1) * nzh + k] : C{(T)0, (T)0};
+                    if (k == 0 || 2 * k == nz) { A.y = (T)0; B.y = (T)0; }
+                } else {
+                    A = s[(2 * p) * nzh + nz - k];
+                    B = hb ? s[(2 * p + 1) * nzh + nz - k] : C{(T)0, (T)0};
+                    A.y = -A.y;
+                    B.y = -B.y;
+                }
+                Z[i] = C{A.x - B.y, A.y + B.x};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int it = tid + i * NT_SP;
+            if (it < nit) {
+                int k;
+                const int p = bynz.div(it, k);
+                s[p * RS + k] = Z[i];
+            }
+        }
+        __syncthreads();
+        fft_lines<C, NT_SP, MAXEL_SP, true, false>(s, npc, nz, RS, 1, twz, r.pl_r8);
+        // rows 2 p0 .. 2 p0 + nrow - 1 of the plane
+        const long o = (long)2 * p0 * nz;
+        if (mode == 0) {
+            const gmut<double> E = as_global_mut(r.E6 + off + o);
+            grouped<NT_SP, 16, double>(nrow * nz, [&](int e) { return creep_set ? 0.0 : E[e]; }, [&](int e, double ev) {
+                int z;
+                const int y = bynz.div(e, z);
+                const double val = (double)sr[2 * ((y >> 1) * RS + z) + (y & 1)];
+                E[e] = creep_set ? cr + (sh + val) : ev + (sh + val);
+            });
+        } else {
+            const gmut<double> W = as_global_mut(r.WF + off + o);
+            for (int e = tid; e < nrow * nz; e += NT_SP) {
+                int z;
+                const int y = bynz.div(e, z);
+                W[e] = (double)sr[2 * ((y >> 1) * RS + z) + (y & 1)];
+            }
+        }
+        if (F) {
+            // diagnostics: the running sum of the Green updates (evolution.c:392)
+            for (int e = tid; e < nrow * nz; e += NT_SP) {
+                int z;
+                const int y = bynz.div(e, z);
+                F[o + e] += (double)sr[2 * ((y >> 1) * RS + z) + (y & 1)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------
 // Line-parallel plane kernels for square power-of-two planes (Ny = Nz = N,
 // N = 16, 32, 64): every line of N points (a packed z-row pair, then a y
 // column of the half spectrum) belongs to a group of L lanes of one wave,
@@ -2895,6 +3115,18 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
         else VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         break;
     }
+    case 5:   // split z and y passes (planes too large for one block); a timed pass binds the start
+              // event to the first launch and the end event to the second
+        if (g_kt0) {
+            hipExtLaunchKernelGGL((k_zfwd_s<T>), dim3(L->n_pfwd), dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab,
+                                  L->map_pfwd);
+            hipExtLaunchKernelGGL((k_y_s<T, false>), dim3(L->n_plane6), dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0,
+                                  L->tab, L->map_plane6, 0);
+        } else {
+            hipLaunchKernelGGL((k_zfwd_s<T>), dim3(L->n_pfwd), dim3(NT_SP), L->pl_lds, st, L->tab, L->map_pfwd);
+            hipLaunchKernelGGL((k_y_s<T, false>), dim3(L->n_plane6), dim3(NT_SP), L->pl_lds, st, L->tab, L->map_plane6, 0);
+        }
+        break;
     default: VLAUNCH((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
@@ -2915,6 +3147,16 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
         else VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
         break;
     }
+    case 5:   // split y and z passes
+        if (g_kt0) {
+            hipExtLaunchKernelGGL((k_y_s<T, true>), dim3(n), dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab, map,
+                                  mode);
+            hipExtLaunchKernelGGL((k_zinv_s<T>), dim3(n), dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0, L->tab, map, mode);
+        } else {
+            hipLaunchKernelGGL((k_y_s<T, true>), dim3(n), dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
+            hipLaunchKernelGGL((k_zinv_s<T>), dim3(n), dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
+        }
+        break;
     default: VLAUNCH((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
     }
 }
@@ -3021,7 +3263,9 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB>, (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 4>,
                          (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB>, (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 2>,
-                         (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 4>};
+                         (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 4>,
+                         (const void *)k_zfwd_s<T>, (const void *)k_y_s<T, false>, (const void *)k_y_s<T, true>,
+                         (const void *)k_zinv_s<T>};
     for (const void *f : fns) {
         // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
         hipFuncAttributes fa;
@@ -3042,10 +3286,11 @@ extern "C" int voce_kernels_prepare(void)
     return (int)e;
 }
 
-// 2^a 3^b 5^c 7^d, 2..128
+// 2^a 3^b 5^c 7^d, 2..512 (planes beyond one block's LDS take the split
+// plane passes; the LDS-tiled x pass holds a 512-point column)
 extern "C" int voce_grid_supported(int n)
 {
-    if (n < 2 || n > 128) return 0;
+    if (n < 2 || n > SP_MAXN) return 0;
     int m = n;
     while (m % 2 == 0) m /= 2;
     while (m % 3 == 0) m /= 3;
@@ -3055,6 +3300,18 @@ extern "C" int voce_grid_supported(int n)
 }
 
 extern "C" int voce_vox_block(void) { return NT_VOX; }
+
+// Default plane variant of an RVE before tuning: the line-parallel kernels
+// where the grid allows them, else the 256- or 512-thread whole-plane
+// kernels, else the split passes; -1 if none fits.
+extern "C" int voce_default_plv(const RveDev *d)
+{
+    if (voce_plane_lds(d, 4) > 0) return 4;
+    if (voce_plane_lds(d, 1) > 0) return 1;
+    if (voce_plane_lds(d, 2) > 0) return 2;
+    if (voce_plane_lds(d, 5) > 0) return 5;
+    return -1;
+}
 
 static const int PL_MAXEL[4] = {MAXEL_PL_T, MAXEL_PL_S, MAXEL_PL_L, MAXEL_PL_M};
 static const int X_MAXEL[3] = {MAXEL_X, MAXEL_XB, MAXEL_XC};
@@ -3088,6 +3345,12 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
         const int n = r->ny;
         if (r->nz != n || (n != 16 && n != 32 && n != 64)) return 0;
         const size_t b = (size_t)el * cb;
+        return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
+    }
+    if (plv == 5) {
+        // split z and y passes: a chunk of lines and one twiddle table
+        if (r->ny > SP_MAXN || r->nz > SP_MAXN) return 0;
+        const size_t b = (size_t)(MAXEL_SP + (r->ny > r->nz ? r->ny : r->nz)) * cb;
         return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
     }
     // an odd Ny packs its last row with a row of zeros: one more row

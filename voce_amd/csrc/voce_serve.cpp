@@ -466,8 +466,7 @@ static int shadow_start(voce_server *s, int k, int src, const double L[9])
     p += al256(d0.ny * cb);
     d.tw_z = p;
     // FP64 launch choices (defaults; audits are not tuned)
-    d.plv = voce_plane_lds(&d, 1) > 0 ? 1 : 2;
-    if (voce_plane_lds(&d, 4) > 0) d.plv = 4;
+    d.plv = voce_default_plv(&d);
     d.xv = 0;
     d.tk = voce_xgreen_tile(d.nx, 0);
     d.pl_r8 = d.x_r8 = 1;
