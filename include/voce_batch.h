@@ -28,8 +28,9 @@ extern "C" {
  * 4: staged through LDS, a block per whole, half or quarter 128-byte column
  * block; 3 and 4 for Nx = 16, 32 or 64 only),
  * x-pass tile (columns per block), and whether radix-8 stages are used in
- * the plane and x passes (variants 3 and 4 have fixed radices).  us_* are the measured launch times (microseconds)
- * of the chosen plane pair (forward + inverse) and x pass. */
+ * the plane and x passes (variants 3 and 4 have fixed radices).  us_* are
+ * the measured launch times (microseconds) of the chosen plane pair
+ * (forward + inverse) and x pass. */
 typedef struct voce_tune_choice {
     int plv, xv, tk, pl_r8, x_r8;
     double us_plane, us_x;
