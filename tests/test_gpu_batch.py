@@ -150,6 +150,9 @@ def test_batch_radix5_radix7_odd_members(tmp_path):
     n = 2
     ra, fa = _run_alone(cases, n, "fp64")
     rb, fb, info = _run_batch(cases, n, "fp64", tune=False)
+    b = voce_amd.Batch(cases, "fp64", tune=False)
+    assert b.member(1).tuning()["plv"] == 5
+    b.close()
     for i, sp in enumerate(specs):
         for k in range(n):
             assert ra[i][k]["iter"] == rb[k][i]["iter"]
@@ -169,16 +172,19 @@ def test_batch_radix5_radix7_odd_members(tmp_path):
 
 
 def test_batch_member_with_split_plane_passes(tmp_path):
-    """A 6 x 256 x 8 member (planes past one block's LDS: the split z and y
+    """A 3 x 256 x 64 member (planes past one block's LDS: the split z and y
     passes) batched with a 16^3 one: each equals its stand-alone run bit for
     bit and matches the oracle."""
-    specs = [inputs.RveSpec(d, 4, 1300 + i, np.zeros((3, 3))) for i, d in enumerate([(16, 16, 16), (6, 256, 8)])]
+    specs = [inputs.RveSpec(d, 4, 1300 + i, np.zeros((3, 3))) for i, d in enumerate([(16, 16, 16), (3, 256, 64)])]
     for sp in specs:
         sp.velgrad = np.diag([1e-3, -0.5e-3, -0.5e-3])
     cases = inputs.load_specs(str(tmp_path / "b"), specs)
     n = 2
     ra, fa = _run_alone(cases, n, "fp64")
     rb, fb, info = _run_batch(cases, n, "fp64", tune=False)
+    b = voce_amd.Batch(cases, "fp64", tune=False)
+    assert b.member(1).tuning()["plv"] == 5
+    b.close()
     for i, sp in enumerate(specs):
         for k in range(n):
             assert ra[i][k]["iter"] == rb[k][i]["iter"]

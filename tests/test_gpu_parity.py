@@ -5,6 +5,7 @@ tolerances stated at each check.  MIXED bar: macro stress within 1e-4 of FP64.""
 import numpy as np
 import pytest
 
+import voce_amd
 from voce_amd import inputs
 from gpu_helpers import gpu_run, oracle_run, rel, say, write
 
@@ -145,7 +146,6 @@ def test_noninteger_exponent_aborts_like_reference(tmp_path):
     # reference exits with 35; the library returns 35 from the same increment
     c = inputs.Case(dims=(8, 8, 8), n_grains=4, seed=33, n_steps=1, material=inputs.Material(nsrs=12.5))
     path, base = write(tmp_path, c)
-    import voce_amd
     with pytest.raises(RuntimeError, match="code 35"):
         oracle_run(path, base, 1)
     s = voce_amd.Solver.from_case(voce_amd.load_case(path, base))
@@ -246,19 +246,26 @@ def test_fp64_parity_split_plane_passes_forced(tmp_path, dims):
     compare(tmp_path, c, 2, tuning={"plv": 5})
 
 
-@pytest.mark.parametrize("dims", [(4, 8, 200), (6, 256, 8), (256, 6, 4), (5, 144, 150)])
-def test_fp64_parity_dimensions_above_128(tmp_path, dims):
-    # a dimension beyond 128: planes past one block's LDS take the split
-    # passes (the default choice there), Nx = 256 the LDS-tiled x pass with a
-    # 2-column tile; 200 = 8 x 5^2, 144 = 16 x 9, 150 = 2 x 3 x 5^2
+@pytest.mark.parametrize("dims,plv", [((4, 8, 200), 1), ((6, 256, 8), 1), ((256, 6, 4), 1), ((5, 144, 150), 5),
+                                      ((3, 256, 64), 5)])
+def test_fp64_parity_dimensions_above_128(tmp_path, dims, plv):
+    # a dimension beyond 128, default launch choices: planes that still fit
+    # one block's LDS run whole (256-point lines included), larger ones (144 x
+    # 76 and 256 x 33 half-spectrum points) take the split passes; Nx = 256
+    # runs the LDS-tiled x pass with a 2-column tile; 200 = 8 x 5^2,
+    # 144 = 16 x 9, 150 = 2 x 3 x 5^2
     c = inputs.Case(dims=dims, n_grains=5, seed=37, n_steps=2)
+    path, base = write(tmp_path, c)
+    s = voce_amd.Solver.from_case(voce_amd.load_case(path, base), "fp64")
+    assert s.tuning()["plv"] == plv
+    s.close()
     compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "new_position"))
 
 
 def test_mixed_split_plane_passes_against_oracle(tmp_path):
-    c = inputs.Case(dims=(6, 256, 8), n_grains=5, seed=41, n_steps=2)
+    c = inputs.Case(dims=(3, 256, 64), n_grains=5, seed=41, n_steps=2)
     path, base = write(tmp_path, c)
-    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",))
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning={"plv": 5})
     o, of = oracle_run(path, base, 2, fields=("sig",))
     for k in range(2):
         r = rel(g[k]["sig_avg"], o[k]["sig_avg"])

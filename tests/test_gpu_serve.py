@@ -323,3 +323,34 @@ def test_fp64_server_with_audits(tmp_path):
     aud = [r["audit_sig_rel"] for r in res if r["audited"]]
     say("fp64 audits", aud)
     assert len(aud) == 5 and max(aud) < 1e-10
+
+
+def test_served_split_plane_id_equals_context(tmp_path):
+    """An id whose planes take the split z and y passes (3 x 256 x 64) served
+    beside a 16^3 id: its increments equal the same increments on a
+    stand-alone context, and FP64 audits of it report the mixed error."""
+    specs = [inputs.RveSpec(d, 4, 7100 + i, np.zeros((3, 3))) for i, d in enumerate([(3, 256, 64), (16, 16, 16)])]
+    cases = inputs.load_specs(str(tmp_path), specs)
+    s = _server(cases)
+    assert s.rve(0).tuning()["plv"] == 5
+    rng = inputs.SplitMix64(29)
+    reqs = []
+    for t in range(8):
+        reqs.append((t % 2, inputs.request_velgrad(rng), t % 4 == 0, t))
+    for rid, L, au, tag in reqs:
+        s.submit(rid, L, au, tag)
+    served = _by_tag(s.drain())
+    c = voce_amd.Solver.from_case(cases[0], "mixed")
+    c.apply_tuning(s.rve(0).tuning())
+    for rid, L, au, tag in reqs:
+        if rid != 0:
+            continue
+        c.set_velgrad(L)
+        r = served[tag]
+        _same(c.increment() | {"tag": tag}, r)
+        if au:
+            say("audit", tag, r["audited"], r["audit_sig_rel"])
+            assert r["audited"] and r["audit_sig_rel"] < 1e-4
+    assert np.array_equal(c.field("sig"), s.rve(0).field("sig"))
+    c.close()
+    s.close()
