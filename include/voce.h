@@ -49,7 +49,7 @@ extern "C" {
 #define VOCE_ERR_MESH_INDEX 1139         /* init.c:420-423 */
 #define VOCE_ERR_ARG (-1)
 #define VOCE_ERR_FILE (-2)
-#define VOCE_ERR_UNSUPPORTED_GRID (-3)
+#define VOCE_ERR_UNSUPPORTED_GRID (-3)   /* a dimension not 2^a 3^b 5^c 7^d in 2..512 */
 #define VOCE_ERR_DEVICE (-4)
 #define VOCE_ERR_NOMEM (-5)
 #define VOCE_ERR_NOT_LOADED (-6)
