@@ -1491,22 +1491,23 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 // ------------------------------------------------------------------
 // x pass staged through LDS (x-pass variant 4, Nx = 16, 32 or 64).  A block
 // owns TK columns of one 128-byte column block (all of its TB columns, a
-// half or a quarter): for each component the
-// block's elements (kx, column) form one contiguous run of Nx * TK complex
-// values, so the spectrum and skew-sum loads and stores of a lane are those
-// of an element-wise kernel (a wave touches whole lines), where variant 3
-// reads each line once per column.  In LDS, element (c, i, j) (component c,
-// line index i, column j of the block) has the row-major index
-// a = (c Nx + i) TK + j, moved within its 128-byte chunk of TB elements to
-// a ^ f(a / TB) with f(m) = (m ^ m >> s1 ^ m >> s2 ^ m >> s3) mod TB: the
-// shifts, per element size, Nx and TK, are ones under which the lanes that
-// one LDS cycle serves (the DFT groups' strided line accesses and the
-// point-wise accesses) fall in distinct banks (2 ways at most for FP32 and
-// Nx = 16, 32), found by search over all the kernel's access patterns with
-// 32 banks of 4 bytes.  Phases, separated by
-// block barriers: load -> forward line DFTs (waves as in variant 3: groups
-// of L lanes per line) -> Green operator per k point -> inverse line DFTs
-// -> store.
+// half or a quarter).  For each component the block's elements (kx, column)
+// form one contiguous run of Nx * TK complex values, so the spectrum and
+// skew-sum loads and stores of a lane are those of an element-wise kernel
+// (a wave touches whole lines), where variant 3 reads each line once per
+// column.  Phases, separated by block barriers: load -> forward line DFTs
+// (waves as in variant 3: groups of L lanes per line) -> Green operator per
+// k point -> inverse line DFTs -> store.
+//
+// In LDS, element (c, i, j) (component c, line index i, column j of the
+// block) has the row-major index a = (c Nx + i) TK + j, moved within its
+// 128-byte chunk of TB elements to a ^ f(a / TB), with
+// f(m) = (m ^ m >> s1 ^ m >> s2 ^ m >> s3) mod TB (a zero shift is no term;
+// SW packs s1, s2, s3 as hex digits).  The shifts, per element size, Nx and
+// TK, are ones under which the lanes that one LDS cycle serves (the DFT
+// groups' strided line accesses and the point-wise accesses) fall in
+// distinct banks, 2 ways at most for FP32 at Nx = 16, 32; they were found by
+// search over all the kernel's access patterns with 32 banks of 4 bytes.
 // ------------------------------------------------------------------
 template <typename T, int NX, int TK> struct XC {
     using C = typename cplx_of<T>::t;
@@ -1516,7 +1517,7 @@ template <typename T, int NX, int TK> struct XC {
     static constexpr int WPB = TK / CPW, NT = 64 * WPB;
     static_assert(TK == TB || 2 * TK == TB || 4 * TK == TB, "a block holds a column block, a half or a quarter");
     static_assert(TK % CPW == 0, "whole columns per wave");
-    // the swizzle shifts (0: no term)
+    // the swizzle shifts s1 s2 s3 as hex digits (0: no term)
     static constexpr int SW = sizeof(C) == 16
         ? (NX == 16 ? (TK == 4 ? 0x012 : 0x002) : NX == 32 ? 0x013 : (TK == 8 ? 0x003 : TK == 4 ? 0x002 : 0x001))
         : (NX == 16 ? (TK == 8 ? 0x001 : 0x012) : NX == 32 ? (TK == 16 ? 0x013 : TK == 8 ? 0x136 : 0x023) : (TK == 16 ? 0x003 : 0x013));
