@@ -601,7 +601,9 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
 // y pass transforms each kz column in place.  An odd Ny pairs its last row
 // with a row of zeros, as in the whole-plane kernels.
 // ------------------------------------------------------------------
-constexpr int NT_SP = 256, MAXEL_SP = 4096;
+// 2048 elements per chunk (32 KB of FP64): four blocks per CU; 4096 (two
+// blocks) and 1024 (more chunks per block) measured slower at 256^3
+constexpr int NT_SP = 256, MAXEL_SP = 2048;
 constexpr int SP_MAXN = 512;   // largest Ny, Nz of the split passes
 
 template <typename C>
