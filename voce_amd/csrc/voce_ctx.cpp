@@ -475,14 +475,15 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
         }
     }
     if (su->n_phases == 1) { pcs[1] = pcs[0]; pcs[1].gas = 1; pcs[1].nsys = 0; }
-    // the Newton kernel's flow rule skips the exactly-zero Schmid components
-    // of the standard FCC slip systems (the skipped terms add exact zeros)
+    // the Newton kernel evaluates the flow rule of the standard FCC slip
+    // systems in a Walsh form whose constants are those of voce_fcc_schmid:
+    // taken only when the phase's own Schmid vectors equal them to rounding
     for (int p = 0; p < 2; ++p) {
         PhaseConst &pc = pcs[p];
         int ok = !pc.gas && pc.nsys == 12;
         for (int sy = 0; ok && sy < 12; ++sy)
             for (int i = 0; i < 5; ++i)
-                if ((pc.px[sy][i] != 0.0) != (((voce_fcc_mask(sy) >> i) & 1u) != 0)) ok = 0;
+                if (fabs(pc.px[sy][i] - voce_fcc_schmid(sy, i)) > 1e-14) ok = 0;
         pc.fcc12 = ok;
     }
 
@@ -598,6 +599,15 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
     creep /= 6;
     voigt_to_full(su->stress, Sc);
     const double dt = su->time_step;
+    for (int p = 0; p < 2; ++p) {
+        PhaseConst &pc = pcs[p];
+        const double a = 0.5 / sqrt(3.0), k = dt * pc.gam0 * pc.nsr;
+        pc.wk[0] = pc.gam0 * a;
+        pc.wk[1] = pc.gam0 * 0.5;
+        pc.wk[2] = k / 12.0;
+        pc.wk[3] = k * (0.5 * a);
+        pc.wk[4] = k * 0.25;
+    }
 
     // initial stress per orientation (InitFields, init.c:494-534)
     double DGA[3][3];

@@ -49,8 +49,8 @@ struct PhaseConst {
     int gas;
     int two_crss;   // 1 when forward and backward CRSS are stored separately
     int int_n;      // 1 when nsrs is an integer in [1, 64]
-    int fcc12;      // 1 when the twelve Schmid vectors have the zero pattern of the
-                    // {111}<110> systems in the order of Cu_p.sx (see voce_fcc_mask)
+    int fcc12;      // 1 when the twelve Schmid vectors are those of the {111}<110>
+                    // systems in the order and signs of Cu_p.sx (voce_fcc_schmid)
     int pad_;
     double nsr, gam0, tau0f, tau0b, tau1, thet0, thet1;
     double px[12][5];      // crystal-frame Schmid vectors Schm_xt (init.c:217-224)
@@ -58,6 +58,10 @@ struct PhaseConst {
     double bn[12][3], bb[12][3];   // slip normal and direction whose product is px
     double rn[12][3], rb[12][3];   // the slip vectors update_orient rotates (constitutive.c:202-203)
     double hard[12][12];
+    // factors of the Walsh form of the flow rule (fcc12 only): edot scale
+    // gam0/(2 sqrt 3), gam0/2; Newton-matrix scale dt gam0 n times 1/12,
+    // 1/(4 sqrt 3), 1/4
+    double wk[5];
 };
 
 // Macro (RVE-wide) state, resident in device memory.  Updated by the
@@ -172,10 +176,19 @@ int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_or
 int voce_grid_supported(int n);
 #define VOCE_N_PLV 6                // plane-kernel variants (4: line-parallel square power-of-two planes,
                                     // 5: split z and y passes)
-// nonzero B-basis components (bit a) of the crystal-frame Schmid vectors of
-// the twelve {111}<110> systems in the order of the reference's Cu_p.sx
-// (three systems per plane)
-static inline unsigned voce_fcc_mask(int s) { return (s % 3 == 0) ? 27u : (s % 3 == 1) ? 23u : 13u; }
+// B-basis component a of the crystal-frame Schmid vector of system s of the
+// twelve {111}<110> systems in the order and signs of the reference's
+// Cu_p.sx.  System s = 3p + j is direction j of plane p; over the four planes
+// each (a, j) component is kappa_aj times a Walsh row, (-1)^popcount(w_aj & p),
+// with kappa one of +-1/(2 sqrt 3), -1/sqrt 3, -1/2 or 0.
+static inline double voce_fcc_schmid(int s, int a)
+{
+    static const int W[5][3] = {{3, 0, 0}, {3, 0, 0}, {0, 1, 1}, {1, 0, 2}, {0, 3, 0}};
+    static const double K[5][3] = {{1, -1, -2}, {-1, -1, 0}, {0, 1, 1}, {1, 0, -1}, {1, 1, 0}};
+    const int p = s / 3, j = s % 3, w = W[a][j] & p;
+    const double unit = (a == 1) ? 0.5 : 0.5 / sqrt(3.0);
+    return ((w == 1 || w == 2) ? -1.0 : 1.0) * K[a][j] * unit;
+}
 #define VOCE_N_XV 5                 // x-pass variants (3: line-parallel, 4: staged through LDS; Nx 16, 32, 64)
 int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);

@@ -2288,76 +2288,83 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
                 for (int b2 = a; b2 < 5; ++b2) { D[q] += t * P[b2]; ++q; }
             }
         };
-        // the standard FCC systems: components that are exactly zero (the
-        // pattern voce_fcc_mask gives, checked on the host) are left out of
-        // the sums, which is exact
-        auto rates_m = [&](auto Mt, const double *P, double icf, double icb, double &rss1, double &rss2) {
-            constexpr unsigned M = decltype(Mt)::value;
-            double rss0 = 0.0;
-#pragma unroll
-            for (int a = 0; a < 5; ++a)
-                if ((M >> a) & 1u) rss0 += P[a] * sig[a];
-            const double itc = (rss0 < 0) ? icb : icf;
-            const double rr = rss0 * itc;
-            double pn1, pn;
-            if (PM == PM_N20) {
-                const double ar = fabs(rr);
-                const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
-                pn1 = (ar * x2) * x16;
-                pn = pn1 * ar;
-            } else if (PM == PM_INT) {
-                const double ar = fabs(rr);
-                pn1 = ipow(ar, ni - 1);
-                pn = pn1 * ar;
-            } else {
-                pn1 = fabs(pow(rr, nsr - 1.));
-                pn = fabs(pow(rr, nsr));
-            }
-            rss1 = g0 * nsr * pn1 * itc;
-            rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
-        };
-        auto accum_m = [&](auto Mt, const double *P, double rss1, double rss2) {
-            constexpr unsigned M = decltype(Mt)::value;
-#pragma unroll
-            for (int a = 0; a < 5; ++a)
-                if ((M >> a) & 1u) ed[a] += P[a] * rss2;
-            int q = 0;
-#pragma unroll
-            for (int a = 0; a < 5; ++a) {
-                const double t = P[a] * rss1;
-#pragma unroll
-                for (int b2 = a; b2 < 5; ++b2) {
-                    if (((M >> a) & (M >> b2) & 1u) != 0) D[q] += t * P[b2];
-                    ++q;
-                }
-            }
-        };
-        auto pair_m = [&](int s, auto M0, auto M1) {
-            RELOAD_BARRIER();
-            double P0[5], P1[5];
-#pragma unroll
-            for (int a = 0; a < 5; ++a) {
-                P0[a] = pc->px[s][a];
-                P1[a] = pc->px[s + 1][a];
-            }
-            const double f0 = lds_ic[s][threadIdx.x], f1 = lds_ic[s + 1][threadIdx.x];
-            const double b0 = TWO ? lds_ic[TWO ? 12 + s : 0][threadIdx.x] : f0;
-            const double b1 = TWO ? lds_ic[TWO ? 13 + s : 0][threadIdx.x] : f1;
-            double r10, r20, r11, r21;
-            rates_m(M0, P0, f0, b0, r10, r20);
-            rates_m(M1, P1, f1, b1, r11, r21);
-            accum_m(M0, P0, r10, r20);
-            accum_m(M1, P1, r11, r21);
-        };
         int s = 0;
         if (FCC) {
-            // masks 27, 23, 13 per plane (voce_fcc_mask), two systems per trip
-            pair_m(0, UMask<27u>{}, UMask<23u>{});
-            pair_m(2, UMask<13u>{}, UMask<27u>{});
-            pair_m(4, UMask<23u>{}, UMask<13u>{});
-            pair_m(6, UMask<27u>{}, UMask<23u>{});
-            pair_m(8, UMask<13u>{}, UMask<27u>{});
-            pair_m(10, UMask<23u>{}, UMask<13u>{});
+            // The standard FCC systems (voce_fcc_schmid, host-checked): system
+            // 3p + j is direction j of plane p, and over the four planes each
+            // Schmid component is a constant times a Walsh row (A = ++++,
+            // B = +-+-, C = ++--, E = +--+).  So the four resolved shears of
+            // direction j are a Walsh transform of at most three scaled
+            // stress components, and edot = sum_s P_s v_s and
+            // D = sum_s P_s P_s^T u_s need only the Walsh transforms over the
+            // planes of v = r^n sgn r and u = r^(n-1) / tau_c: a product of
+            // two Walsh rows is a Walsh row.  The factors gam0 and n (and dt
+            // for D) are the host's wk.
+            constexpr double AL = 0.28867513459481287;   // 1 / (2 sqrt 3)
+            const double a0 = AL * sig[0], b1 = 0.5 * sig[1], a2 = AL * sig[2], a3 = AL * sig[3], a4 = AL * sig[4];
+            double Wu[3][4], Wv[3][4];
+            auto dir = [&](int j, double r0, double r1, double r2, double r3) {
+                const double rs[4] = {r0, r1, r2, r3};
+                double u[4], vv[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const int sy = 3 * p + j;
+                    const double icf = lds_ic[sy][threadIdx.x];
+                    const double icb = TWO ? lds_ic[TWO ? 12 + sy : 0][threadIdx.x] : icf;
+                    const double itc = (rs[p] < 0) ? icb : icf;
+                    const double rr = rs[p] * itc;
+                    const double ar = fabs(rr);
+                    const double x2 = ar * ar, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+                    const double pn1 = (ar * x2) * x16;
+                    u[p] = pn1 * itc;
+                    vv[p] = copysign(pn1 * ar, rr);
+                }
+                const double su = u[0] + u[1], du = u[0] - u[1], tu = u[2] + u[3], eu = u[2] - u[3];
+                Wu[j][0] = su + tu;
+                Wu[j][1] = du + eu;
+                Wu[j][2] = su - tu;
+                Wu[j][3] = du - eu;
+                const double sv = vv[0] + vv[1], dv = vv[0] - vv[1], tv = vv[2] + vv[3], ev = vv[2] - vv[3];
+                Wv[j][0] = sv + tv;
+                Wv[j][1] = dv + ev;
+                Wv[j][2] = sv - tv;
+                Wv[j][3] = dv - ev;
+            };
+            {   // j = 0: Walsh coefficients A = a4, B = a3, E = a0 - b1
+                const double E = a0 - b1, sAB = a4 + a3, dAB = a4 - a3;
+                dir(0, sAB + E, dAB - E, sAB - E, dAB + E);
+            }
+            {   // j = 1: A = -(a0 + b1), B = a2, E = a4
+                const double A = -(a0 + b1), sAB = A + a2, dAB = A - a2;
+                dir(1, sAB + a4, dAB - a4, sAB - a4, dAB + a4);
+            }
+            {   // j = 2: A = -2 a0, B = a2, C = -a3
+                const double m = -(2.0 * a0 + a3), n = a3 - 2.0 * a0;
+                dir(2, m + a2, m - a2, n + a2, n - a2);
+            }
+            const double ka = pc->wk[0], kb = pc->wk[1];
+            ed[0] = ka * fma(-2.0, Wv[2][0], Wv[0][3] - Wv[1][0]);
+            ed[1] = -kb * (Wv[0][3] + Wv[1][0]);
+            ed[2] = ka * (Wv[1][1] + Wv[2][1]);
+            ed[3] = ka * (Wv[0][1] - Wv[2][2]);
+            ed[4] = ka * (Wv[0][0] + Wv[1][3]);
+            // D (upper triangle, row major) without its factors
+            const double s01 = Wu[0][0] + Wu[1][0];
+            D[0] = fma(4.0, Wu[2][0], s01);
+            D[1] = Wu[1][0] - Wu[0][0];
+            D[2] = -fma(2.0, Wu[2][1], Wu[1][1]);
+            D[3] = fma(2.0, Wu[2][2], Wu[0][2]);
+            D[4] = Wu[0][3] - Wu[1][3];
+            D[5] = s01;
+            D[6] = -Wu[1][1];
+            D[7] = -Wu[0][2];
+            D[8] = -(Wu[0][3] + Wu[1][3]);
+            D[9] = Wu[1][0] + Wu[2][0];
+            D[10] = -Wu[2][3];
+            D[11] = Wu[1][2];
+            D[12] = Wu[0][0] + Wu[2][0];
+            D[13] = Wu[0][1];
+            D[14] = s01;
             s = 12;
         }
         for (; s + 1 < nsys; s += 2) {
@@ -2417,7 +2424,16 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
 #pragma unroll
             for (int a = 0; a < 5; ++a)
 #pragma unroll
-                for (int b2 = a; b2 < 5; ++b2) { K[lt6(b2, a)] += dt * D[q]; ++q; }
+                for (int b2 = a; b2 < 5; ++b2) {
+                    if (FCC) {
+                        // factors of the Walsh form: dt gam0 n / 12, / (4 sqrt 3), / 4
+                        const double f = pc->wk[(q == 1 || (q >= 6 && q <= 8)) ? 3 : (q == 5 ? 4 : 2)];
+                        K[lt6(b2, a)] += f * D[q];
+                    } else {
+                        K[lt6(b2, a)] += dt * D[q];
+                    }
+                    ++q;
+                }
         }
         double x[6];
         if (ldlt_solve6_packed(K, g, x)) return 35;
