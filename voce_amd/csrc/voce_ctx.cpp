@@ -568,6 +568,18 @@ int voce_ctx_build(const voce_setup *su, const voce_build_opts *opt, voce_ctx **
             otab[VOCE_OTAB * o + 36 + i] = 0.5 * ((S066[a][b] + Sloc[36 * o + i]) + (S066[b][a] + Sloc[36 * o + b * 6 + a]));
         }
         for (int i = 0; i < 9; ++i) otab[VOCE_OTAB * o + 72 + i] = Rtab[o][i];   // initial TranMat
+        // Cubic elasticity (ElastStiffnessMatrix, init.c:9-49) keeps the
+        // hydrostatic B-basis component an eigenvector of every C_gr, of their
+        // mean C066 and of the inverses, so K_c couples it to the deviatoric
+        // components by rounding only.  The Newton kernel solves the two parts
+        // apart (a 5x5 deviatoric system and one division); a coupling beyond
+        // rounding is refused.
+        const double *K = &otab[VOCE_OTAB * o + 36];
+        for (int j = 0; j < 5; ++j)
+            if (!(fabs(K[30 + j]) <= 1e-12 * fabs(K[35]) && fabs(K[6 * j + 5]) <= 1e-12 * fabs(K[35]))) {
+                free(ms0);
+                return VOCE_ERR_ARG;
+            }
     }
 
     // boundary conditions (init.c:713-745)

@@ -108,12 +108,13 @@ struct RveDev {
     // `fluctuation`, evolution.c:392) that new_position reads: B-basis
     // components 2, 3, 4, 6 of the symmetric part, then skew (2,1) and (0,2)
     double *FLU;
-    // workspace: K_c = C066^-1 + S_local of the voxel's orientation-table
-    // row in the crystal frame of its TranMat (Q6^T K_c Q6, Q6 the B-basis
-    // rotation), packed lower triangle, written by the first Newton launch
-    // of each increment (TranMat is fixed until the close-out).  Tiles of 64
-    // voxels, component-major inside a tile: component q of voxel v at
-    // ((v / 64) * 21 + q) * 64 + v % 64 (VOCE_KCV_COMP * voce_kcv_voxels(V) doubles)
+    // workspace: the deviatoric 5x5 block of K_c = C066^-1 + S_local of the
+    // voxel's orientation-table row in the crystal frame of its TranMat
+    // (Q^T K_c Q, Q the B-basis rotation), packed lower triangle, written by
+    // the first Newton launch of each increment (TranMat is fixed until the
+    // close-out).  Tiles of 64 voxels, component-major inside a tile:
+    // component q of voxel v at ((v / 64) * 15 + q) * 64 + v % 64
+    // (VOCE_KCV_COMP * voce_kcv_voxels(V) doubles)
     double *KCV;
     uint32_t *ORI;
     uint8_t *PH;
@@ -158,7 +159,7 @@ struct LaunchSet {
 // voxels the KCV array is allocated for (whole tiles of 64), and doubles per voxel
 static inline long voce_kcv_voxels(long V) { return (V + 63) & ~63L; }
 #ifndef VOCE_KCV_COMP
-#define VOCE_KCV_COMP 21
+#define VOCE_KCV_COMP 15
 #endif
 
 #ifdef __cplusplus

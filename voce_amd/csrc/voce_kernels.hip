@@ -2135,17 +2135,18 @@ __device__ __forceinline__ double rcp_refined(double d)
 // Packed lower triangle of a symmetric 6x6: (i, j), j <= i, at i (i + 1) / 2 + j.
 __device__ __forceinline__ constexpr int lt6(int i, int j) { return i * (i + 1) / 2 + j; }
 
-// Solve K x = g for symmetric positive definite K (packed lower triangle,
-// factorised in place: L below the diagonal, the pivots on it) by LDL^T.
-// Returns 35 (the reference's singular-matrix code) if a pivot is not
+// Solve K x = g for symmetric positive definite N x N K (packed lower
+// triangle, factorised in place: L below the diagonal, the pivots on it) by
+// LDL^T.  Returns 35 (the reference's singular-matrix code) if a pivot is not
 // positive, which a positive definite matrix never gives.
-__device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6], double x[6])
+template <int N>
+__device__ __forceinline__ int ldlt_solve_packed(double K[N * (N + 1) / 2], const double g[N], double x[N])
 {
-    double iD[6];
+    double iD[N];
     int bad = 0;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        double v[6];
+    for (int j = 0; j < N; ++j) {
+        double v[N];
 #pragma unroll
         for (int k = 0; k < j; ++k) v[k] = K[lt6(j, k)] * K[lt6(k, k)];
         double d = K[lt6(j, j)];
@@ -2155,35 +2156,36 @@ __device__ __forceinline__ int ldlt_solve6_packed(double K[21], const double g[6
         K[lt6(j, j)] = d;
         iD[j] = rcp_refined(d);
 #pragma unroll
-        for (int i = j + 1; i < 6; ++i) {
+        for (int i = j + 1; i < N; ++i) {
             double t = K[lt6(i, j)];
 #pragma unroll
             for (int k = 0; k < j; ++k) t -= K[lt6(i, k)] * v[k];
             K[lt6(i, j)] = t * iD[j];
         }
     }
-    double y[6];
+    double y[N];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < N; ++i) {
         double t = g[i];
 #pragma unroll
         for (int k = 0; k < i; ++k) t -= K[lt6(i, k)] * y[k];
         y[i] = t;
     }
 #pragma unroll
-    for (int i = 5; i >= 0; --i) {
+    for (int i = N - 1; i >= 0; --i) {
         double t = y[i] * iD[i];
 #pragma unroll
-        for (int k = i + 1; k < 6; ++k) t -= K[lt6(k, i)] * x[k];
+        for (int k = i + 1; k < N; ++k) t -= K[lt6(k, i)] * x[k];
         x[i] = t;
     }
     return bad ? 35 : 0;
 }
 
-// K_c (row-major 6x6, symmetric) in the crystal frame of the TranMat whose
-// B-basis rotation is Q: Q6^T K_c Q6 with Q6 = diag(Q, 1) (the hydrostatic
-// component is rotation invariant), written as its packed lower triangle,
-// in tiles of 64 voxels (component q of voxel v at ((v / 64) 21 + q) 64 + v % 64).
+// The deviatoric block of K_c (row-major 6x6, symmetric) in the crystal frame
+// of the TranMat whose B-basis rotation is Q: Q^T K_c Q, written as its packed
+// lower triangle, in tiles of 64 voxels (component q of voxel v at
+// ((v / 64) 15 + q) 64 + v % 64).  The hydrostatic row and column couple by
+// rounding only (cubic elasticity, checked on the host) and are not kept.
 __device__ __forceinline__ double *kcv_voxel(const RveDev &r, long v) { return r.KCV + (v >> 6) * (VOCE_KCV_COMP * 64) + (v & 63); }
 
 __device__ __forceinline__ void store_kc_crystal(const RveDev &r, long v, const double *Kc, const double Q[5][5])
@@ -2208,14 +2210,6 @@ __device__ __forceinline__ void store_kc_crystal(const RveDev &r, long v, const 
             for (int a = 0; a < 5; ++a) t += Q[a][i] * M[a][j];
             dst[lt6(i, j) * 64] = t;
         }
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-        double t = 0.0;
-#pragma unroll
-        for (int a = 0; a < 5; ++a) t += Kc[30 + a] * Q[a][j];
-        dst[lt6(5, j) * 64] = t;
-    }
-    dst[lt6(5, 5) * 64] = Kc[35];
 }
 
 // Forms of the power law x^(n-1), x^n of the flow rule: n = 20 (a fixed
@@ -2230,15 +2224,18 @@ enum { PM_N20 = 0, PM_INT = 1, PM_REAL = 2 };
 template <unsigned V> struct UMask { static constexpr unsigned value = V; };
 
 template <int NT, bool TWO, int PM, bool FCC>
-__device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double signorm, cptr<PhaseConst> pc, int nsys,
-                                              double dt, gptr<double> kcv, double (*lds_w)[NT], double (*lds_old)[NT],
-                                              double (*lds_ic)[NT])
+__device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double tol2, double h6, cptr<PhaseConst> pc,
+                                              int nsys, double dt, gptr<double> kcv, double (*lds_w)[NT],
+                                              double (*lds_old)[NT], double (*lds_ic)[NT])
 {
     const double nsr = pc->nsr, g0 = pc->gam0;
     const int ni = (int)nsr;
-    double erral = 1E-6;   // 10 * erroral (evolution.c:58-61)
+    // the loop runs while erral = |d sigma| / |lambda| > 1e-7, tested as
+    // |d sigma|^2 > tol2 = 1e-14 |lambda|^2; it starts at erral = 1e-6
+    // (evolution.c:58-63)
+    bool more = true;
     int iterl = 0;
-    while ((iterl < 100) && (fabs(erral) > 1E-7)) {
+    while ((iterl < 100) && more) {
         ++iterl;
         RELOAD_BARRIER();
 #pragma unroll
@@ -2408,15 +2405,19 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
         // dt D x nearly cancel) into the converged stress.
         // One triangle of K_c,x (21 loads, issued together) serves the
         // residual and the factorisation.
-        double K[21];
+        // The hydrostatic component decouples (cubic elasticity: K_c has no
+        // coupling beyond rounding, D has no hydrostatic row), so its Newton
+        // step lands on h6 = -w6 / K_c66 and the rest is a 5x5 system in the
+        // deviatoric block of K_c,x (15 loads, issued together).
+        double K[15];
 #pragma unroll
-        for (int q = 0; q < 21; ++q) K[q] = kcv[q * 64];
-        double g[6];
+        for (int q = 0; q < 15; ++q) K[q] = kcv[q * 64];
+        double g[5];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            double t = lds_w[i][threadIdx.x] + (i < 5 ? dt * ed[i] : 0.0);
+        for (int i = 0; i < 5; ++i) {
+            double t = lds_w[i][threadIdx.x] + dt * ed[i];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
+            for (int j = 0; j < 5; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
             g[i] = t;
         }
         {
@@ -2435,16 +2436,17 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
                     ++q;
                 }
         }
-        double x[6];
-        if (ldlt_solve6_packed(K, g, x)) return 35;
-        double cn = 0.0;
+        double x[5];
+        if (ldlt_solve_packed<5>(K, g, x)) return 35;
+        double cn = (h6 - sig[5]) * (h6 - sig[5]);
+        sig[5] = h6;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
+        for (int i = 0; i < 5; ++i) {
             const double so = sig[i];
             sig[i] -= x[i];
             cn += (sig[i] - so) * (sig[i] - so);
         }
-        erral = sqrt(cn) / signorm;
+        more = cn > tol2;
     }
     return 0;
 }
@@ -2485,6 +2487,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             // it, and a load behind a branch or a use costs a round trip of
             // its own.  The CRSS arrays hold twelve systems per voxel.
             double cq[12], Rm[9], s0[6], w[6];
+            const double kc66 = tab[36 + 35];   // K_c, hydrostatic diagonal entry
             {
                 const gptr<double> cf = as_global(r.CRSS + v);
 #pragma unroll
@@ -2517,7 +2520,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             // the constant crystal-frame ones (Schm_xt, init.c:217-224) and
             // K_c,x is the per-voxel table the close-out keeps; the loop
             // never touches R.
-            double sig[6], signorm = 0.0;
+            double sig[6], lam2 = 0.0;
             {
                 if (msc->iter == 0) {
                     // first iteration of the increment: K_c in the frame of
@@ -2527,7 +2530,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                     store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
                 }
 #pragma unroll
-                for (int i = 0; i < 6; ++i) signorm += s0[i] * s0[i];
+                for (int i = 0; i < 6; ++i) lam2 += s0[i] * s0[i];
                 // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
                 // part of the compliance-form residual below, kept in LDS (a
                 // per-lane register extension)
@@ -2554,7 +2557,7 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                 sig[5] = s0[5];
                 lds_w[5][threadIdx.x] = w[5];
             }
-            signorm = sqrt(signorm);
+            const double tol2 = 1E-14 * lam2, h6 = -w[5] / kc66;
             const gptr<double> kcv = as_global(kcv_voxel(r, v));
             double ed[5];
             // the power law's form is uniform per phase: one loop per form
@@ -2562,13 +2565,13 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             const int pm = pc->int_n ? (ni == 20 ? PM_N20 : PM_INT) : PM_REAL;
             int err;
             if (pm == PM_N20 && pc->fcc12)
-                err = newton_crystal<NT, TWO, PM_N20, true>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_N20, true>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else if (pm == PM_N20)
-                err = newton_crystal<NT, TWO, PM_N20, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_N20, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else if (pm == PM_INT)
-                err = newton_crystal<NT, TWO, PM_INT, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_INT, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else
-                err = newton_crystal<NT, TWO, PM_REAL, false>(sig, ed, signorm, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_REAL, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             if (err) ms->error = err;
             // back to the sample frame: stress, the stress of the last
             // flow-rule evaluation and its strain rate
