@@ -9,7 +9,8 @@
 //   S6     6 x V f64   Cauchy stress, B basis
 //   P6     6 x V f64   stress at which the last NR flow-rule evaluation ran
 //                      (the "pre-update" stress; Edot and gamdot follow from
-//                      it exactly, evolution.c:67,125-133)
+//                      it exactly, evolution.c:67,125-133), in the crystal
+//                      frame of TranMat R
 //   EP5    5 x V f64   plastic strain, B basis (the 6th component is 0)
 //   ED5    5 x V f64   plastic strain rate of the increment, B basis
 //   CRSS  12 x V f64   forward CRSS; a second 12 x V block holds the

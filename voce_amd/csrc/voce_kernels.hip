@@ -1726,6 +1726,32 @@ __device__ __forceinline__ void basis_rotation_col(const double R[9], int b, dou
     q[4] = 2. * RSQ2 * m[5];
 }
 
+// Deviatoric B-basis components of R T R^T (TO_SAMPLE: crystal -> sample)
+// or of R^T T R (sample -> crystal), for the traceless symmetric T whose
+// components are a: the 3x3 products, cheaper than forming Q when one or two
+// tensors are rotated.  The trace of the result is zero, so its (2,2) entry
+// is taken as -(00 + 11).
+template <bool TO_SAMPLE>
+__device__ __forceinline__ void rot_dev(const double R[9], const double a[5], double out[5])
+{
+    const double d0 = RSQ2 * a[0], d1 = RSQ6 * a[1];
+    const double e12 = RSQ2 * a[2], e02 = RSQ2 * a[3], e01 = RSQ2 * a[4];
+    const double T[3][3] = {{-d0 - d1, e01, e02}, {e01, d0 - d1, e12}, {e02, e12, 2.0 * d1}};
+    auto A = [&](int i, int j) { return TO_SAMPLE ? R[i * 3 + j] : R[j * 3 + i]; };
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[i][j] = A(i, 0) * T[0][j] + A(i, 1) * T[1][j] + A(i, 2) * T[2][j];
+    auto Tp = [&](int i, int j) { return M[i][0] * A(j, 0) + M[i][1] * A(j, 1) + M[i][2] * A(j, 2); };
+    const double t00 = Tp(0, 0), t11 = Tp(1, 1);
+    out[0] = RSQ2 * (t11 - t00);
+    out[1] = -3.0 * RSQ6 * (t00 + t11);
+    out[2] = 2. * RSQ2 * Tp(1, 2);
+    out[3] = 2. * RSQ2 * Tp(0, 2);
+    out[4] = 2. * RSQ2 * Tp(0, 1);
+}
+
 __device__ __forceinline__ void basis_rotation(const double R[9], double Q[5][5])
 {
 #pragma unroll
@@ -1787,22 +1813,15 @@ __device__ __forceinline__ double ipow(double x, int n)
     return r;
 }
 
-// StrainRate_eval (constitutive.c:4-72) evaluated in the crystal frame:
-// tau_s = p_s . sx with sx = Q^T sigma; edot = Q ex, D = Q Dx Q^T with
-// ex = sum p_s gamdot_s and Dx = sum p_s p_s^T tau_x_s.  ed and D (packed
-// upper 5x5) are returned in the sample frame.  gd / tx receive gamdot and
-// tau_x when non-null (12 entries each).
-__device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[5][5], const PhaseConst *pc, int nsys,
-                                            const double cf[12], const double cb[12], double ed[5],
-                                            double D[15], double *gd, double *tx)
+// StrainRate_eval (constitutive.c:4-72) in the crystal frame: tau_s = p_s . sx
+// for the crystal-frame stress sx, ex = sum p_s gamdot_s (crystal frame; the
+// sample-frame rate is Q ex).  gd / tx receive gamdot and tau_x when non-null
+// (12 entries each).
+__device__ __forceinline__ void flow_rule_x(const double sx[6], const PhaseConst *pc, int nsys, const double cf[12],
+                                            const double cb[12], double ex[5], double *gd, double *tx)
 {
-    double sx[5];
 #pragma unroll
-    for (int b = 0; b < 5; ++b) sx[b] = Q[0][b] * sig[0] + Q[1][b] * sig[1] + Q[2][b] * sig[2] + Q[3][b] * sig[3] + Q[4][b] * sig[4];
-    double ex[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double Dx[15];
-#pragma unroll
-    for (int i = 0; i < 15; ++i) Dx[i] = 0.0;
+    for (int a = 0; a < 5; ++a) ex[a] = 0.0;
     const double nsr = pc->nsr, g0 = pc->gam0;
     const int intn = pc->int_n;
     const int ni = (int)nsr;
@@ -1826,42 +1845,9 @@ __device__ __forceinline__ void flow_rule_q(const double sig[6], const double Q[
             const double rss2 = g0 * pn * ((double)(2 * (rr > 0) - 1));
             if (gd) gd[s] = rss2;
             if (tx) tx[s] = rss1;
-            double t[5];
 #pragma unroll
-            for (int a = 0; a < 5; ++a) {
-                ex[a] += p[a] * rss2;
-                t[a] = p[a] * rss1;
-            }
-            int q = 0;
-#pragma unroll
-            for (int a = 0; a < 5; ++a)
-#pragma unroll
-                for (int b = a; b < 5; ++b) { Dx[q] += t[a] * p[b]; ++q; }
+            for (int a = 0; a < 5; ++a) ex[a] += p[a] * rss2;
         }
-    }
-#pragma unroll
-    for (int a = 0; a < 5; ++a) ed[a] = Q[a][0] * ex[0] + Q[a][1] * ex[1] + Q[a][2] * ex[2] + Q[a][3] * ex[3] + Q[a][4] * ex[4];
-#pragma unroll
-    for (int i = 0; i < 15; ++i) D[i] = 0.0;
-    // D = Q Dx Q^T, accumulated one column of Dx Q^T at a time
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        double m[5];
-#pragma unroll
-        for (int b = 0; b < 5; ++b) {
-            double acc = 0.0;
-#pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                const int lo = c < e ? c : e, hi = c < e ? e : c;
-                acc += Dx[lo * 5 - (lo * (lo - 1)) / 2 + (hi - lo)] * Q[b][e];
-            }
-            m[b] = acc;
-        }
-        int q = 0;
-#pragma unroll
-        for (int a = 0; a < 5; ++a)
-#pragma unroll
-            for (int b = a; b < 5; ++b) { D[q] += Q[a][c] * m[b]; ++q; }
     }
 }
 
@@ -2224,9 +2210,9 @@ enum { PM_N20 = 0, PM_INT = 1, PM_REAL = 2 };
 template <unsigned V> struct UMask { static constexpr unsigned value = V; };
 
 template <int NT, bool TWO, int PM, bool FCC>
-__device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double tol2, double h6, cptr<PhaseConst> pc,
-                                              int nsys, double dt, gptr<double> kcv, double (*lds_w)[NT],
-                                              double (*lds_old)[NT], double (*lds_ic)[NT])
+__device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], double glast[5], double tol2, double h6,
+                                              cptr<PhaseConst> pc, int nsys, double dt, gptr<double> kcv,
+                                              double (*lds_w)[NT], double (*lds_old)[NT], double (*lds_ic)[NT])
 {
     const double nsr = pc->nsr, g0 = pc->gam0;
     const int ni = (int)nsr;
@@ -2419,6 +2405,7 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
 #pragma unroll
             for (int j = 0; j < 5; ++j) t += K[i >= j ? lt6(i, j) : lt6(j, i)] * sig[j];
             g[i] = t;
+            glast[i] = t;
         }
         {
             int q = 0;
@@ -2522,13 +2509,6 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
             // never touches R.
             double sig[6], lam2 = 0.0;
             {
-                if (msc->iter == 0) {
-                    // first iteration of the increment: K_c in the frame of
-                    // this increment's TranMat
-                    double Q[5][5];
-                    basis_rotation(Rm, Q);
-                    store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
-                }
 #pragma unroll
                 for (int i = 0; i < 6; ++i) lam2 += s0[i] * s0[i];
                 // w = (eps_p_t - eps) - S0 lambda (S0 = C066^-1): the constant
@@ -2540,76 +2520,65 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
 #pragma unroll
                     for (int j = 0; j < 6; ++j) w[i] -= S0b[i * 6 + j] * s0[j];
                 }
-                // Q^T, one column of Q at a time
+                // to the crystal frame
+                double wx[5];
+                rot_dev<false>(Rm, s0, sig);
+                rot_dev<false>(Rm, w, wx);
 #pragma unroll
-                for (int b = 0; b < 5; ++b) {
-                    double q[5];
-                    basis_rotation_col(Rm, b, q);
-                    double ts = 0.0, tw = 0.0;
-#pragma unroll
-                    for (int a = 0; a < 5; ++a) {
-                        ts += q[a] * s0[a];
-                        tw += q[a] * w[a];
-                    }
-                    sig[b] = ts;
-                    lds_w[b][threadIdx.x] = tw;
-                }
+                for (int b = 0; b < 5; ++b) lds_w[b][threadIdx.x] = wx[b];
                 sig[5] = s0[5];
                 lds_w[5][threadIdx.x] = w[5];
             }
             const double tol2 = 1E-14 * lam2, h6 = -w[5] / kc66;
+            if (msc->iter == 0) {
+                // first iteration of the increment: K_c in the frame of this
+                // increment's TranMat
+                double Q[5][5];
+                basis_rotation(Rm, Q);
+                store_kc_crystal(r, v, r.SLOC + (long)ori * VOCE_OTAB + 36, Q);
+            }
             const gptr<double> kcv = as_global(kcv_voxel(r, v));
-            double ed[5];
+            double ed[5], glast[5];
             // the power law's form is uniform per phase: one loop per form
             const int ni = (int)pc->nsr;
             const int pm = pc->int_n ? (ni == 20 ? PM_N20 : PM_INT) : PM_REAL;
             int err;
             if (pm == PM_N20 && pc->fcc12)
-                err = newton_crystal<NT, TWO, PM_N20, true>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_N20, true>(sig, ed, glast, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else if (pm == PM_N20)
-                err = newton_crystal<NT, TWO, PM_N20, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_N20, false>(sig, ed, glast, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else if (pm == PM_INT)
-                err = newton_crystal<NT, TWO, PM_INT, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_INT, false>(sig, ed, glast, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             else
-                err = newton_crystal<NT, TWO, PM_REAL, false>(sig, ed, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
+                err = newton_crystal<NT, TWO, PM_REAL, false>(sig, ed, glast, tol2, h6, pc, nsys, dt, kcv, lds_w, lds_old, lds_ic);
             if (err) ms->error = err;
-            // back to the sample frame: stress, the stress of the last
-            // flow-rule evaluation and its strain rate
-            double sig_old[6], wsa[6];
+            // Back to the sample frame: the stress, the stress of the last
+            // flow-rule evaluation and the Newton residual g there.  The
+            // error partial |eps_tot(sigma_old) - eps| (evolution.c:103-114)
+            // is that of g(sigma_old) - C066^-1 (sigma_old - lambda), since
+            // g = K_c sigma + w + dt edot, K_c = C066^-1 + S_local and
+            // w = eps_p - eps - C066^-1 lambda.  The stress of the last
+            // flow-rule evaluation is stored in the crystal frame (only the
+            // close-out reads it).
+            double osa[6], gsa[6];
             RELOAD_BARRIER();
             {
                 double Rm[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) Rm[i] = as_global(r.R)[i * V + v];
-                double sx[5], ox[5], ex[5], wx[5];
+                double sx[5], ox[6];
 #pragma unroll
-                for (int i = 0; i < 5; ++i) {
-                    sx[i] = sig[i];
-                    ox[i] = lds_old[i][threadIdx.x];
-                    ex[i] = ed[i];
-                    wx[i] = lds_w[i][threadIdx.x];
-                    sig[i] = sig_old[i] = ed[i] = wsa[i] = 0.0;
-                }
-                sig_old[5] = lds_old[5][threadIdx.x];
-                wsa[5] = lds_w[5][threadIdx.x];
-                // Q t, one column of Q at a time
+                for (int i = 0; i < 6; ++i) ox[i] = lds_old[i][threadIdx.x];
 #pragma unroll
-                for (int b = 0; b < 5; ++b) {
-                    double q[5];
-                    basis_rotation_col(Rm, b, q);
+                for (int i = 0; i < 5; ++i) sx[i] = sig[i];
 #pragma unroll
-                    for (int a = 0; a < 5; ++a) {
-                        sig[a] += q[a] * sx[b];
-                        sig_old[a] += q[a] * ox[b];
-                        ed[a] += q[a] * ex[b];
-                        wsa[a] += q[a] * wx[b];
-                    }
-                }
+                for (int i = 0; i < 6; ++i) as_global_mut(r.P6)[i * V + v] = ox[i];
+                gsa[5] = fma(kc66, ox[5], lds_w[5][threadIdx.x]);
+                osa[5] = ox[5];
+                rot_dev<true>(Rm, sx, sig);
+                rot_dev<true>(Rm, ox, osa);
+                rot_dev<true>(Rm, glast, gsa);
             }
-            // error partials of the last NR iteration (evolution.c:103-114):
-            // |sigma - lambda| and |eps_tot(sigma_old) - eps|, the latter as
-            // w + C066^-1 lambda + dt edot + S_local sigma_old (w holds
-            // eps_p - eps - C066^-1 lambda)
             double cs = 0.0, ce = 0.0;
             RELOAD_BARRIER();
             double lam[6];
@@ -2618,17 +2587,14 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
                 lam[i] = as_global(r.S6)[i * V + v];
                 cs += (sig[i] - lam[i]) * (sig[i] - lam[i]);
                 as_global_mut(r.S6)[i * V + v] = sig[i];
-                as_global_mut(r.P6)[i * V + v] = sig_old[i];
             }
             {
                 const cptr<double> S0b = msc->S066;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    double t = wsa[i] + (i < 5 ? ed[i] * dt : 0.0);
+                    double t = gsa[i];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) t += S0b[i * 6 + j] * lam[j];
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) t += tab[i * 6 + j] * sig_old[j];
+                    for (int j = 0; j < 6; ++j) t += S0b[i * 6 + j] * (lam[j] - osa[j]);
                     ce += t * t;
                 }
             }
@@ -2826,9 +2792,10 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
             for (int i = 0; i < 9; ++i) Rm[i] = r.R[i * V + v];
             double Q[5][5];
             basis_rotation(Rm, Q);
-            double sp[6];
+            // the stress of the last flow-rule evaluation, crystal frame
+            double sx6[6];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) sp[i] = r.P6[i * V + v];
+            for (int i = 0; i < 6; ++i) sx6[i] = r.P6[i * V + v];
             double *crf = r.CRSS;
             double *crb = pc->two_crss ? r.CRSSB : r.CRSS;
             double cf[12], cb[12];
@@ -2837,10 +2804,18 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
                 cf[s] = s < nsys ? crf[s * V + v] : 1.0;
                 cb[s] = s < nsys ? crb[s * V + v] : 1.0;
             }
-            double ed[5], D[15], gd[12], tx[12];
+            double ex[5], ed[5], gd[12], tx[12];
 #pragma unroll
             for (int s = 0; s < 12; ++s) gd[s] = tx[s] = 0.0;
-            flow_rule_q(sp, Q, pc, nsys, cf, cb, ed, D, gd, tx);
+            flow_rule_x(sx6, pc, nsys, cf, cb, ex, gd, tx);
+            // strain rate and the pre-update stress in the sample frame
+            double sp[6];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                ed[a] = Q[a][0] * ex[0] + Q[a][1] * ex[1] + Q[a][2] * ex[2] + Q[a][3] * ex[3] + Q[a][4] * ex[4];
+                sp[a] = Q[a][0] * sx6[0] + Q[a][1] * sx6[1] + Q[a][2] * sx6[2] + Q[a][3] * sx6[3] + Q[a][4] * sx6[4];
+            }
+            sp[5] = sx6[5];
             double ep[5];
 #pragma unroll
             for (int i = 0; i < 5; ++i) ep[i] = r.EP5[i * V + v];
