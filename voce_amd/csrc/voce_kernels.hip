@@ -1862,39 +1862,47 @@ __device__ __forceinline__ double dpp_f64(double x)
                             __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, false));
 }
 
-// Sum of x over the 64 lanes of a wave, complete in lane 63: DPP moves in a
-// fixed order (quad swaps, half-row and row mirrors, row broadcasts), so
-// the result does not depend on timing and no LDS round trip is taken.
-__device__ __forceinline__ double wave_sum_last(double x)
+// Sum of x over each 32-lane half of a wave, complete in lanes 31 and 63:
+// DPP moves in a fixed order (quad swaps, half-row and row mirrors, a row
+// broadcast), so the result does not depend on timing.
+__device__ __forceinline__ double halfwave_sum_last(double x)
 {
     x += dpp_f64<0xB1, 0xf>(x);    // quad_perm [1,0,3,2]
     x += dpp_f64<0x4E, 0xf>(x);    // quad_perm [2,3,0,1]
     x += dpp_f64<0x141, 0xf>(x);   // row_half_mirror
     x += dpp_f64<0x140, 0xf>(x);   // row_mirror: every lane holds its row's sum
     x += dpp_f64<0x142, 0xa>(x);   // row_bcast:15 into rows 1 and 3
-    x += dpp_f64<0x143, 0xc>(x);   // row_bcast:31 into rows 2 and 3
     return x;
 }
 
-// DUP: out[NV .. NV + 5] repeat out[2 .. 7] (with one crystalline phase the
-// phase-1 stress sums are the stress sums)
+// Block sums of NV values per lane, stored by out[0 .. NV).  The values go
+// through LDS in red (NV rows of NT, column = thread: a thread writes only
+// its own column, so red may alias per-lane arrays the thread is done with),
+// then a 32-lane group sums one value: lane j of the group adds entries
+// j, j + 32, j + 64, ... in that order (consecutive lanes, consecutive
+// doubles), and a DPP tree over the group adds the lanes.  The order is
+// fixed.  DUP: out[NV .. NV + 5] repeat out[2 .. 7] (with one crystalline
+// phase the phase-1 stress sums are the stress sums).
 template <int NT, int NV, bool DUP = false>
-__device__ __forceinline__ void block_reduce_store(double (&vals)[NV], double *out)
+__device__ __forceinline__ void block_reduce_store(const double (&vals)[NV], double (*red)[NT], double *out)
 {
-    __shared__ double red[NT / 64][NV];
+    static_assert(NT % 32 == 0, "whole 32-lane groups");
 #pragma unroll
-    for (int i = 0; i < NV; ++i) vals[i] = wave_sum_last(vals[i]);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 63)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) red[w][i] = vals[i];
+    for (int i = 0; i < NV; ++i) red[i][threadIdx.x] = vals[i];
     __syncthreads();
-    if (threadIdx.x < NV) {
-        double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < NT / 64; ++k) acc += red[k][threadIdx.x];
-        out[threadIdx.x] = acc;
-        if (DUP && threadIdx.x >= 2) out[threadIdx.x + 6] = acc;
+    for (int base = 0; base < NV * 32; base += NT) {
+        const int t = base + (int)threadIdx.x, i = t >> 5, j = t & 31;
+        double x = 0.0;
+        if (i < NV) {
+#pragma unroll
+            for (int e = 0; e < NT / 32; ++e) x += red[i][j + 32 * e];
+        }
+        x = halfwave_sum_last(x);
+        if (i < NV && j == 31) {
+            out[i] = x;
+            if (DUP && i >= 2) out[i + 6] = x;
+        }
     }
 }
 
@@ -2446,9 +2454,10 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
     if (!iter_active(ms)) return;
     // per-lane values that stay constant through the Newton loop live in LDS
     // (component-major: conflict-free), keeping the register file for the loop
-    __shared__ double lds_w[6][NT];     // w = (eps_p - eps) - C066^-1 lambda
-    __shared__ double lds_old[6][NT];   // stress at the last flow-rule evaluation
-    __shared__ double lds_ic[TWO ? 24 : 12][NT];   // 1 / CRSS, forward (and backward)
+    __shared__ double lds_all[12 + (TWO ? 24 : 12)][NT];
+    double (*lds_w)[NT] = lds_all;         // w = (eps_p - eps) - C066^-1 lambda
+    double (*lds_old)[NT] = lds_all + 6;   // stress at the last flow-rule evaluation
+    double (*lds_ic)[NT] = lds_all + 12;   // 1 / CRSS, forward (and backward)
     const long V = r.V;
     const long v = (long)lb * NT + threadIdx.x;
     double acc[14];
@@ -2612,9 +2621,9 @@ __global__ __launch_bounds__(NT, TWO ? 2 : VOCE_NR_WAVES) void k_nr(const RveDev
         double a8[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) a8[i] = acc[i];
-        block_reduce_store<NT, 8, true>(a8, r.part + (long)lb * 16);
+        block_reduce_store<NT, 8, true>(a8, lds_all, r.part + (long)lb * 16);
     } else {
-        block_reduce_store<NT, 14>(acc, r.part + (long)lb * 16);
+        block_reduce_store<NT, 14>(acc, lds_all, r.part + (long)lb * 16);
     }
     if (lb == 0 && threadIdx.x == 0) ms->fin_pending = 1;   // read after this kernel ends
 }
@@ -2967,7 +2976,8 @@ __global__ __launch_bounds__(NT) void k_closeout(const RveDev *__restrict__ rtab
             }
         }
     }
-    block_reduce_store<NT, 10>(acc, r.part + (long)lb * 16);
+    __shared__ double red[10][NT];
+    block_reduce_store<NT, 10>(acc, red, r.part + (long)lb * 16);
 }
 
 // increment finaliser (evolution.c:438-498)
