@@ -58,8 +58,9 @@ def kernel_bytes_per_voxel(name, nx, ny, nz, f32, two_crss):
         return 2 * 6 * c * spec + 2 * 3 * c * spec
     if name == "plane_inv":                  # read 6 spectra, read + write E (6 f64)
         return 6 * c * spec + 2 * 6 * 8
-    if name == "nr":                         # read E, S, EP, CRSS, R, crystal-frame K_c, ORI, PH; write S, P
-        return (6 + 6 + 5 + 12 * (2 if two_crss else 1) + 9 + 21) * 8 + 4 + 1 + (6 + 6) * 8
+    if name == "nr":                         # read E, S, EP, CRSS, R, crystal-frame K_c (deviatoric
+        # triangle), ORI, PH; write S, P
+        return (6 + 6 + 5 + 12 * (2 if two_crss else 1) + 9 + 15) * 8 + 4 + 1 + (6 + 6) * 8
     return 0.0
 
 
