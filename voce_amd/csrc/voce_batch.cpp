@@ -117,6 +117,8 @@ static int build_group_maps(voce_batch *b, Group &g)
     L.plv = d0.plv;
     L.p2_n = d0.ny;
     L.p2_nx = d0.nx;
+    L.sp_ny = d0.ny;
+    L.sp_nz = d0.nz;
     L.p2_tk = d0.tk;
     L.xv = d0.xv;
     L.pl_lds = pl_lds;

@@ -289,6 +289,8 @@ int voce_sync_tab(voce_ctx *c)
     L.plv = d.plv;
     L.p2_n = d.ny;
     L.p2_nx = d.nx;
+    L.sp_ny = d.ny;
+    L.sp_nz = d.nz;
     L.p2_tk = d.tk;
     L.xv = d.xv;
     L.pl_lds = voce_plane_lds(&d, d.plv);

@@ -153,6 +153,7 @@ struct LaunchSet {
     int plv, xv;                 // kernel variants (same for every RVE of the set)
     int p2_n;                    // plv 4: the square plane size ny = nz of the set
     int p2_nx, p2_tk;            // xv 3: nx and the x-pass tile of the set
+    int sp_ny, sp_nz;            // plv 5: ny and nz of the set (chunks of the split passes)
     size_t pl_lds, x_lds;        // dynamic LDS: the largest over the set's RVEs
     int fft_f32, uniform_phase, two_crss;
 };

@@ -593,9 +593,10 @@ __global__ __launch_bounds__(NT) void k_plane_inv(const RveDev *__restrict__ rta
 // ------------------------------------------------------------------
 // Split plane passes (plane variant 5), for planes too large for one
 // block's LDS (any Ny, Nz up to 512).  The z and y transforms of a plane are
-// two launches; a block owns one (component, x-plane) as in the whole-plane
-// kernels (so the block maps are the same) and walks it in chunks of lines
-// that fit MAXEL_SP elements of LDS.  Between the two launches the half
+// two launches; a block owns one chunk of lines (that fit MAXEL_SP elements
+// of LDS) of one (component, x-plane): blockIdx.x maps to the (component,
+// x-plane) as in the whole-plane kernels (so the block maps are the same) and
+// blockIdx.y is the chunk.  Between the two launches the half
 // spectrum of z row y sits in the spectrum array at column y * nzh + kz (the
 // column-blocked layout of the final spectrum, with y in place of ky); the
 // y pass transforms each kz column in place.  An odd Ny pairs its last row
@@ -621,7 +622,7 @@ __global__ __launch_bounds__(NT_SP) void k_zfwd_s(const RveDev *__restrict__ rta
     using C = typename cplx_of<T>::t;
     RVE_PROLOGUE(lb);
     if (lb == 6 * r.nx) {
-        finalize_block<NT_SP>(r);
+        if (blockIdx.y == 0) finalize_block<NT_SP>(r);
         return;
     }
     if (!iter_active(r.ms)) return;
@@ -637,7 +638,9 @@ __global__ __launch_bounds__(NT_SP) void k_zfwd_s(const RveDev *__restrict__ rta
     T *sr = (T *)s;
     const FastDiv bynz(nz), bynzh(nzh);
     const int ppc = MAXEL_SP / RS;   // row pairs per chunk
-    for (int p0 = 0; p0 < np; p0 += ppc) {
+    {
+        const int p0 = (int)blockIdx.y * ppc;
+        if (p0 >= np) return;
         const int npc = min(ppc, np - p0);
         const int nrow = min(2 * npc, ny - 2 * p0);
         grouped<NT_SP, 16, double>(nrow * nz, [&](int e) { return src[(long)2 * p0 * nz + e]; },
@@ -681,7 +684,9 @@ __global__ __launch_bounds__(NT_SP) void k_y_s(const RveDev *__restrict__ rtab, 
     const int c = lb / nx, x = lb - c * nx;
     C *sp = (C *)r.spec + (long)c * r.nspec;
     const int kpc = MAXEL_SP / ny;   // kz columns per chunk
-    for (int k0 = 0; k0 < nzh; k0 += kpc) {
+    {
+        const int k0 = (int)blockIdx.y * kpc;
+        if (k0 >= nzh) return;
         const int nk = min(kpc, nzh - k0);
         const FastDiv bynk(nk);
         // element e = y nk + j: consecutive lanes on consecutive columns
@@ -734,7 +739,9 @@ __global__ __launch_bounds__(NT_SP) void k_zinv_s(const RveDev *__restrict__ rta
     if (r.FLU && mode == 1 && c < 2) F = r.FLU + (long)(4 + c) * r.V + (long)x * ny * nz;
     const T *sr = (const T *)s;
     const int ppc = MAXEL_SP / RS;
-    for (int p0 = 0; p0 < np; p0 += ppc) {
+    {
+        const int p0 = (int)blockIdx.y * ppc;
+        if (p0 >= np) return;
         const int npc = min(ppc, np - p0);
         const int nrow = min(2 * npc, ny - 2 * p0);
         // the chunk's half-spectrum rows, row y - 2 p0 at (y - 2 p0) nzh
@@ -3102,6 +3109,19 @@ constexpr int NT_XC = 64, MAXEL_XC = 768;
 // the queue's launch gaps.  Null outside timed launches, which then go
 // through the plain launch (graph capture included).
 thread_local hipEvent_t g_kt0 = nullptr, g_kt1 = nullptr;
+// chunks of the split passes for the set's planes: row pairs of the z passes
+// (a pair is 2 (Nz/2 + 1) elements), kz columns of the y pass (Ny elements each)
+static inline unsigned sp_chunks_z(const LaunchSet *L)
+{
+    const int np = (L->sp_ny + 1) / 2, ppc = MAXEL_SP / (2 * (L->sp_nz / 2 + 1));
+    return (unsigned)((np + ppc - 1) / ppc);
+}
+static inline unsigned sp_chunks_y(const LaunchSet *L)
+{
+    const int nzh = L->sp_nz / 2 + 1, kpc = MAXEL_SP / L->sp_ny;
+    return (unsigned)((nzh + kpc - 1) / kpc);
+}
+
 #define VLAUNCH(kern, grid, block, shm, st, ...)                                           \
     do {                                                                                    \
         if (g_kt0) hipExtLaunchKernelGGL(kern, grid, block, shm, st, g_kt0, g_kt1, 0, __VA_ARGS__); \
@@ -3122,18 +3142,19 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
         else VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         break;
     }
-    case 5:   // split z and y passes (planes too large for one block); a timed pass binds the start
-              // event to the first launch and the end event to the second
+    case 5: {   // split z and y passes (planes too large for one block), a block per chunk; a
+                // timed pass binds the start event to the first launch and the end event to the second
+        const dim3 gz(L->n_pfwd, sp_chunks_z(L)), gy(L->n_plane6, sp_chunks_y(L));
         if (g_kt0) {
-            hipExtLaunchKernelGGL((k_zfwd_s<T>), dim3(L->n_pfwd), dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab,
-                                  L->map_pfwd);
-            hipExtLaunchKernelGGL((k_y_s<T, false>), dim3(L->n_plane6), dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0,
-                                  L->tab, L->map_plane6, 0);
+            hipExtLaunchKernelGGL((k_zfwd_s<T>), gz, dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab, L->map_pfwd);
+            hipExtLaunchKernelGGL((k_y_s<T, false>), gy, dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0, L->tab,
+                                  L->map_plane6, 0);
         } else {
-            hipLaunchKernelGGL((k_zfwd_s<T>), dim3(L->n_pfwd), dim3(NT_SP), L->pl_lds, st, L->tab, L->map_pfwd);
-            hipLaunchKernelGGL((k_y_s<T, false>), dim3(L->n_plane6), dim3(NT_SP), L->pl_lds, st, L->tab, L->map_plane6, 0);
+            hipLaunchKernelGGL((k_zfwd_s<T>), gz, dim3(NT_SP), L->pl_lds, st, L->tab, L->map_pfwd);
+            hipLaunchKernelGGL((k_y_s<T, false>), gy, dim3(NT_SP), L->pl_lds, st, L->tab, L->map_plane6, 0);
         }
         break;
+    }
     default: VLAUNCH((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
@@ -3154,16 +3175,17 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
         else VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
         break;
     }
-    case 5:   // split y and z passes
+    case 5: {   // split y and z passes, a block per chunk
+        const dim3 gy(n, sp_chunks_y(L)), gz(n, sp_chunks_z(L));
         if (g_kt0) {
-            hipExtLaunchKernelGGL((k_y_s<T, true>), dim3(n), dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab, map,
-                                  mode);
-            hipExtLaunchKernelGGL((k_zinv_s<T>), dim3(n), dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0, L->tab, map, mode);
+            hipExtLaunchKernelGGL((k_y_s<T, true>), gy, dim3(NT_SP), L->pl_lds, st, g_kt0, nullptr, 0, L->tab, map, mode);
+            hipExtLaunchKernelGGL((k_zinv_s<T>), gz, dim3(NT_SP), L->pl_lds, st, nullptr, g_kt1, 0, L->tab, map, mode);
         } else {
-            hipLaunchKernelGGL((k_y_s<T, true>), dim3(n), dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
-            hipLaunchKernelGGL((k_zinv_s<T>), dim3(n), dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
+            hipLaunchKernelGGL((k_y_s<T, true>), gy, dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
+            hipLaunchKernelGGL((k_zinv_s<T>), gz, dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
         }
         break;
+    }
     default: VLAUNCH((k_plane_inv<T, NT_PL_L, MAXEL_PL_L>), dim3(n), dim3(NT_PL_L), L->pl_lds, st, L->tab, map, mode); break;
     }
 }

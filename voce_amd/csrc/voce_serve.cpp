@@ -528,6 +528,8 @@ static int build_sets(voce_server *s, const std::vector<int> &rows, std::vector<
         L.plv = d0.plv;
         L.p2_n = d0.ny;
         L.p2_nx = d0.nx;
+        L.sp_ny = d0.ny;
+        L.sp_nz = d0.nz;
         L.p2_tk = d0.tk;
         L.xv = d0.xv;
         L.fft_f32 = d0.fft_f32;
