@@ -202,10 +202,11 @@ def test_mixed_precision_within_1e4(tmp_path):
                                          ((32, 12, 16), {"xv": 4, "tk": 4}),
                                          ((16, 16, 16), {"xv": 4, "tk": 2}),
                                          ((32, 12, 16), {"xv": 4, "tk": 2}),
-                                         ((32, 32, 32), {"plv": 4, "xv": 4, "tk": 8})])
+                                         ((32, 32, 32), {"plv": 4, "xv": 4, "tk": 8}),
+                                         ((2, 128, 128), {"plv": 4}), ((3, 128, 128), {"plv": 4})])
 def test_fp64_parity_line_parallel(tmp_path, dims, tuning):
     # plane variant 4 (line-parallel kernels of square power-of-two planes,
-    # any Nx), x-pass variant 3 (line-parallel x pass, Nx = 16, 32, 64, any
+    # any Nx; 128 x 128 planes take 16-point lines per lane), x-pass variant 3 (line-parallel x pass, Nx = 16, 32, 64, any
     # plane) and x-pass variant 4 (staged through LDS, a whole, half or
     # quarter column block per block; 12 x 9 = 108 columns leave a partial
     # last block),
@@ -224,6 +225,19 @@ def test_mixed_line_parallel_planes_against_oracle(tmp_path, tuning):
     for k in range(3):
         r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
         say("incr", k + 1, "mixed (plane variant 4) vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-6
+
+
+def test_mixed_line_parallel_128_planes_against_oracle(tmp_path):
+    # FP32 line-parallel plane kernels of 128 x 128 planes
+    c = inputs.Case(dims=(2, 128, 128), n_grains=8, seed=29, n_steps=2)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning={"plv": 4})
+    o, of = oracle_run(path, base, 2, fields=("sig",))
+    for k in range(2):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed (plane variant 4, 128 x 128) vs oracle", r)
         assert r < 1e-4
     assert rel(gf["sig"], of["sig"]) < 1e-6
 

@@ -187,6 +187,39 @@ template <bool INV, typename C> struct Bfly<8, INV, C> {
     }
 };
 
+template <bool INV, typename C> struct Bfly<16, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        using T = decltype(v[0].x);
+        // radix-2 on pairs (k, k+8), twiddle W16^k, two radix-8 on evens/odds
+        const T c1 = (T)0.92387953251128674, s1 = (T)0.38268343236508977, h = (T)0.70710678118654752;
+        const T wr[8] = {(T)1, c1, h, s1, (T)0, -s1, -h, -c1};
+        const T wi[8] = {(T)0, s1, h, c1, (T)1, c1, h, s1};   // W16^k = wr - i wi (forward)
+        C e[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            e[k] = cadd(v[k], v[k + 8]);
+            o[k] = csub(v[k], v[k + 8]);
+        }
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            if (k == 4) {
+                o[4] = cmul_mi<INV>(o[4]);
+            } else {
+                const C w = C{wr[k], INV ? wi[k] : -wi[k]};
+                o[k] = cmul(o[k], w);
+            }
+        }
+        Bfly<8, INV, C>::run(e);
+        Bfly<8, INV, C>::run(o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[2 * k] = e[k];
+            v[2 * k + 1] = o[k];
+        }
+    }
+};
+
 // Odd-prime butterflies (R = 5, 7) in the symmetric form: with
 // t_j = v_j + v_(R-j), u_j = v_j - v_(R-j) (j = 1..(R-1)/2),
 // X_0 = v_0 + sum t_j, and for k = 1..(R-1)/2
@@ -821,7 +854,7 @@ __global__ __launch_bounds__(NT_SP) void k_zinv_s(const RveDev *__restrict__ rta
 
 // ------------------------------------------------------------------
 // Line-parallel plane kernels for square power-of-two planes (Ny = Nz = N,
-// N = 16, 32, 64): every line of N points (a packed z-row pair, then a y
+// N = 16, 32, 64, 128): every line of N points (a packed z-row pair, then a y
 // column of the half spectrum) belongs to a group of L lanes of one wave,
 // R = N / L points per lane.  A line is transformed by an R-point DFT in
 // registers, the twiddles W_N^(n1 k1), one exchange through the line's own
@@ -835,6 +868,7 @@ template <int N> struct P2Plan;
 template <> struct P2Plan<16> { static constexpr int L = 4, R = 4, NT = 64; };
 template <> struct P2Plan<32> { static constexpr int L = 4, R = 8, NT = 128; };
 template <> struct P2Plan<64> { static constexpr int L = 8, R = 8, NT = 320; };
+template <> struct P2Plan<128> { static constexpr int L = 8, R = 16, NT = 576; };
 
 // Orders the LDS accesses of one wave (no instruction is emitted: the
 // accesses of a wave are performed in program order; this keeps the
@@ -1019,15 +1053,19 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
     // lanes on consecutive pairs), loaded once the spectrum is in: issued at
     // the start, these loads made the y pass wait for them as well (a wait
     // on a load also waits on every load issued before it)
+    // (at N = 128 the 16-point z DFT needs those registers: loaded after it)
     double2 ev[2][R / 2];
-    if (mode == 0 && !creep_set && grp < N / 2) {
-        const double2 *E = (const double2 *)(r.E6 + off);
+    auto load_e = [&]() {
+        if (mode == 0 && !creep_set && grp < N / 2) {
+            const double2 *E = (const double2 *)(r.E6 + off);
 #pragma unroll
-        for (int q = 0; q < R / 2; ++q) {
-            ev[0][q] = gload(E + n1 + L * q);
-            ev[1][q] = gload(E + N / 2 + n1 + L * q);
+            for (int q = 0; q < R / 2; ++q) {
+                ev[0][q] = gload(E + n1 + L * q);
+                ev[1][q] = gload(E + N / 2 + n1 + L * q);
+            }
         }
-    }
+    };
+    if (N <= 64) load_e();
     __syncthreads();
     // z: rows 2p (a) and 2p+1 (b) as one complex row Z = A + iB, A and B
     // Hermitian-extended from their half spectra, inverse DFT along z
@@ -1051,6 +1089,7 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
             v[q] = C{a.x - b.y, a.y + b.x};
         }
         line_dft<C, N, true>(v, tw, n1, [&](int i) { return line + i; });
+        if (N > 64) load_e();
         // back through the line's slots to pairs of consecutive points
         wave_lds_sync();
 #pragma unroll
@@ -3139,7 +3178,8 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
         const int n = L->p2_n;
         if (n == 16) VLAUNCH((k_plane_fwd_p2<T, 16>), dim3(L->n_pfwd), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         else if (n == 32) VLAUNCH((k_plane_fwd_p2<T, 32>), dim3(L->n_pfwd), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
-        else VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else if (n == 64) VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else VLAUNCH((k_plane_fwd_p2<T, 128>), dim3(L->n_pfwd), dim3(P2Plan<128>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         break;
     }
     case 5: {   // split z and y passes (planes too large for one block), a block per chunk; a
@@ -3172,7 +3212,8 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
         const int m = L->p2_n;
         if (m == 16) VLAUNCH((k_plane_inv_p2<T, 16>), dim3(n), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, map, mode);
         else if (m == 32) VLAUNCH((k_plane_inv_p2<T, 32>), dim3(n), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, map, mode);
-        else VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
+        else if (m == 64) VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
+        else VLAUNCH((k_plane_inv_p2<T, 128>), dim3(n), dim3(P2Plan<128>::NT), L->pl_lds, st, L->tab, map, mode);
         break;
     }
     case 5: {   // split y and z passes, a block per chunk
@@ -3284,6 +3325,7 @@ hipError_t set_lds_limits()
                          (const void *)k_plane_fwd_p2<T, 16>, (const void *)k_plane_inv_p2<T, 16>,
                          (const void *)k_plane_fwd_p2<T, 32>, (const void *)k_plane_inv_p2<T, 32>,
                          (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>,
+                         (const void *)k_plane_fwd_p2<T, 128>, (const void *)k_plane_inv_p2<T, 128>,
                          (const void *)k_xgreen_p2<T, 16, 2>, (const void *)k_xgreen_p2<T, 16, 4>,
                          (const void *)k_xgreen_p2<T, 32, 2>, (const void *)k_xgreen_p2<T, 32, 4>,
                          (const void *)k_xgreen_p2<T, 64, 2>, (const void *)k_xgreen_p2<T, 64, 4>,
@@ -3370,9 +3412,9 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
     const int el = r->ny * r->nzh;
     const size_t cb = r->fft_f32 ? 8 : 16;
     if (plv == 4) {
-        // line-parallel variant: square planes of 16, 32 or 64, the plane only
+        // line-parallel variant: square planes of 16, 32, 64 or 128, the plane only
         const int n = r->ny;
-        if (r->nz != n || (n != 16 && n != 32 && n != 64)) return 0;
+        if (r->nz != n || (n != 16 && n != 32 && n != 64 && n != 128)) return 0;
         const size_t b = (size_t)el * cb;
         return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
     }
