@@ -127,6 +127,57 @@ def cpu_baseline(path, base, sample_iters, threads):
     return V * r["iter"] / dt, dt, r["iter"]
 
 
+def run_plane_strain_128(d, steps, warmup):
+    """128^3 / 1000-grain plane-strain compression (velocity gradient
+    diag(1, 0, -1) 1e-3/s, dt 1 s): MIXED, and FP64 on the same seed for the
+    error of the macroscopic stress curve and of the final stress field."""
+    import voce_amd
+    from voce_amd import inputs
+    c = inputs.config_case(3)
+    c.n_steps = steps + warmup
+    d3 = os.path.join(d, "plane_strain_128")
+    os.makedirs(d3, exist_ok=True)
+    log("generating 128^3 / 1000-grain Voronoi mesh in", d3)
+    inputs.write_case(d3, c)
+    cs = voce_amd.load_case(os.path.join(d3, "input"), d3)
+    res, curves, fields = {}, {}, {}
+    for prec in ("mixed", "fp64"):
+        s = voce_amd.Solver.from_case(cs, prec)
+        tuned = s.autotune()
+        for _ in range(warmup):
+            s.increment()
+        s.close()
+        # a fresh context (the tuned choices are kept per process), timed
+        s = voce_amd.Solver.from_case(cs, prec)
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(s.increment())
+        wall = time.perf_counter() - t0
+        it = sum(r["iter"] for r in reps)
+        fields[prec] = s.field("sig")
+        curves[prec] = reps
+        V = s.nvox
+        s.close()
+        res[prec] = {"voxel_iterations_per_s": round(V * it / wall, 1), "fft_iterations": it,
+                     "ms_per_increment": round(wall / steps * 1e3, 3), "wall_s": round(wall, 3),
+                     "fft_launch_choice": tuned}
+        log("128^3 %s: %d increments, %d iterations, %.3f s, %.4g voxel-it/s" % (prec, steps, it, wall, V * it / wall))
+    cs.free()
+    macro = max(float(np.abs(a["sig_avg"] - b["sig_avg"]).max() / np.abs(b["sig_avg"]).max())
+                for a, b in zip(curves["mixed"], curves["fp64"]))
+    fm, ff = fields["mixed"], fields["fp64"]
+    field = float(np.abs(fm - ff).max() / np.abs(ff).max())
+    return {"workload": "single RVE, 128^3 voxels, 1000 Voronoi grains (seed 3), FCC Cu-like, plane-strain "
+                        "compression L = diag(1, 0, -1) 1e-3/s, dt 1 s, Err 1e-5, IterMax 100; timed: increments "
+                        "1-%d (%.0f %% strain) of a fresh context after %d warm-up increments on another"
+                        % (steps, steps * 0.1, warmup),
+            "voxels": V, "increments": steps, "mixed": res["mixed"], "fp64": res["fp64"],
+            "mixed_over_fp64": round(res["mixed"]["voxel_iterations_per_s"] / res["fp64"]["voxel_iterations_per_s"], 3),
+            "macro_stress_max_rel_err_vs_fp64": macro,
+            "sigma_field_max_abs_err_over_max_vs_fp64": field}
+
+
 def run_batched(d, precision, n_rve, steps, log_every=5):
     """Config-4 style workload: n_rve RVEs of uneven size, each with its own
     velocity gradient, advanced `steps` increments together."""
@@ -266,6 +317,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-serve-fp64", action="store_true")
+    ap.add_argument("--ps128-steps", type=int, default=100)
+    ap.add_argument("--no-ps128", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the RVE lives whole on one MI355X and nothing here "
@@ -363,6 +416,9 @@ def main():
                         "ms_per_step": round(mwall / args.steps * 1e3, 3),
                         "macro_stress_max_rel_err_vs_fp64": err}
         log("mixed: %.3f s, %.4g voxel-it/s, macro rel err %.2e" % (mwall, V * miters / mwall, err))
+
+    if not args.no_ps128:
+        out["plane_strain_128"] = run_plane_strain_128(d, args.ps128_steps, 1)
 
     rve_inc_fp64 = out["rve_increments_per_s"]
     if not args.no_batch:

@@ -1537,7 +1537,7 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 }
 
 // ------------------------------------------------------------------
-// x pass staged through LDS (x-pass variant 4, Nx = 16, 32 or 64).  A block
+// x pass staged through LDS (x-pass variant 4, Nx = 16, 32, 64 or 128).  A block
 // owns TK columns of one 128-byte column block (all of its TB columns, a
 // half or a quarter).  For each component the block's elements (kx, column)
 // form one contiguous run of Nx * TK complex values, so the spectrum and
@@ -1545,7 +1545,9 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 // (a wave touches whole lines), where variant 3 reads each line once per
 // column.  Phases, separated by block barriers: load -> forward line DFTs
 // (waves as in variant 3: groups of L lanes per line) -> Green operator per
-// k point -> inverse line DFTs -> store.
+// k point -> inverse line DFTs -> store.  A wave holds one column's six
+// lines from Nx = 64 on, so at Nx = 128 a thread takes two k points,
+// (kx, j) and (kx + 64, j).
 //
 // In LDS, element (c, i, j) (component c, line index i, column j of the
 // block) has the row-major index a = (c Nx + i) TK + j, moved within its
@@ -1561,8 +1563,10 @@ template <typename T, int NX, int TK> struct XC {
     using C = typename cplx_of<T>::t;
     static constexpr int TB = 128 / (int)sizeof(C);                 // columns per 128-byte block
     static constexpr int L = P2Plan<NX>::L, R = P2Plan<NX>::R;
-    static constexpr int CPW = (NX == 64) ? 1 : 2;                   // columns per wave (DFT role)
+    static constexpr int CPW = (NX >= 64) ? 1 : 2;                   // columns per wave (DFT role)
     static constexpr int WPB = TK / CPW, NT = 64 * WPB;
+    static constexpr int PPT = (NX > 64) ? NX / 64 : 1;               // k points per thread
+    static constexpr int KST = NT / TK;                              // kx step between them
     static_assert(TK == TB || 2 * TK == TB || 4 * TK == TB, "a block holds a column block, a half or a quarter");
     static_assert(TK % CPW == 0, "whole columns per wave");
     // the swizzle shifts s1 s2 s3 as hex digits (0: no term)
@@ -1605,7 +1609,9 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     const long cstr = (long)r.ncb * NX * TB;
     // element (kx, col0 + j) of a component at base + kx * TB + j
     const long base = (long)(col0 / TB) * NX * TB + (col0 % TB);
-    // point role: thread p takes k point (kx, column j)
+    // point role: thread p takes the k points (pkx + h KST, column pj),
+    // h < PPT, at offsets pg + h KST TB of a component
+    constexpr int PPT = X::PPT, KST = X::KST;
     const int p = (int)threadIdx.x, pkx = p / TK, pj = p % TK;
     const bool pt = (pkx < NX) && (col0 + pj < ncolt);
     const long pg = base + (long)pkx * TB + pj;
@@ -1620,33 +1626,49 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     // the twiddles first: a wait on a load also waits on every load issued
     // before it, so the DFTs then wait for the spectra only
     C tw[R];
-    {
-        const C *gtw = (const C *)r.tw_x;
+    const C *gtw = (const C *)r.tw_x;
+    auto load_tw = [&]() {
 #pragma unroll
         for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
-    }
+    };
+    load_tw();
     // load
-    C u[3];
+    C u[PPT][3];
+    auto load_u = [&]() {
+#pragma unroll
+        for (int h = 0; h < PPT; ++h)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) u[h][i] = gload(usum + i * cstr + pg + h * KST * TB);
+    };
     if (pt) {
         if (mode == 0) {
-            C a[6];
+            C a[PPT][6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) a[c] = gload(spec + c * cstr + pg);
+            for (int h = 0; h < PPT; ++h)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a[h][c] = gload(spec + c * cstr + pg + h * KST * TB);
             // the running skew sums after the spectra (kept in that order: a
             // wait on a load also waits on the loads issued before it), and
             // on the first iteration too (then not used), since a load under
-            // a branch makes the waits after it count from the shorter path
+            // a branch makes the waits after it count from the shorter path;
+            // at Nx = 128 only after the forward DFT, whose 16-point lines
+            // need those registers
             __asm__ volatile("" ::: "memory");
+            if (PPT == 1) load_u();
 #pragma unroll
-            for (int i = 0; i < 3; ++i) u[i] = gload(usum + i * cstr + pg);
+            for (int h = 0; h < PPT; ++h)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) s[X::slot(c, pkx, pj)] = a[c];
+                for (int c = 0; c < 6; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
         } else {
-            C a[3];
+            C a[PPT][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) a[c] = gload(usum + c * cstr + pg);
+            for (int h = 0; h < PPT; ++h)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s[X::slot(c, pkx, pj)] = a[c];
+                for (int c = 0; c < 3; ++c) a[h][c] = gload(usum + c * cstr + pg + h * KST * TB);
+#pragma unroll
+            for (int h = 0; h < PPT; ++h)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
             // keeps this tail apart from the other branch's (a shared tail
             // would take this path's shorter wait counts)
             __asm__ volatile("" ::: "memory");
@@ -1664,9 +1686,11 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
 #pragma unroll
             for (int o = 0; o < R; ++o) *line(line_out_index<NX>(n1, o)) = v[o];
         }
+        if (PPT > 1 && pt) load_u();
         __syncthreads();
-        if (pt) {
-            const int kx = pkx, col = col0 + pj;
+#pragma unroll
+        for (int h = 0; h < PPT; ++h) if (pt) {
+            const int kx = pkx + h * KST, col = col0 + pj;
             const int ky = col / nzh, kz = col - ky * nzh;
             C a[6];
 #pragma unroll
@@ -1700,8 +1724,11 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
             for (int c = 0; c < 6; ++c) s[X::slot(c, kx, pj)] = d[c];
             // sum of the skew parts over the increment (lattice spin)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) gstore(usum + i * cstr + pg, first ? wk[i] : cadd(u[i], wk[i]));
+            for (int i = 0; i < 3; ++i) gstore(usum + i * cstr + pg + h * KST * TB, first ? wk[i] : cadd(u[h][i], wk[i]));
         }
+        // at Nx = 128 the 16 twiddles are not held through the Green
+        // operator: loaded again (from the cache) for the inverse DFT
+        if (NX > 64) load_tw();
         __syncthreads();
     }
     // inverse x DFT (mode 1: of the skew sums, into SPEC[0..2])
@@ -1718,10 +1745,14 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     if (pt) {
         if (mode == 0) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) gstore(spec + c * cstr + pg, s[X::slot(c, pkx, pj)]);
+            for (int h = 0; h < PPT; ++h)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gstore(spec + c * cstr + pg, s[X::slot(c, pkx, pj)]);
+            for (int h = 0; h < PPT; ++h)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
         }
     }
 }
@@ -3265,7 +3296,8 @@ void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
     case 4:   // x pass staged through LDS (the grid's nx)
         if (L->p2_nx == 16) launch_xgreen_c<T, 16>(L, st, mode);
         else if (L->p2_nx == 32) launch_xgreen_c<T, 32>(L, st, mode);
-        else launch_xgreen_c<T, 64>(L, st, mode);
+        else if (L->p2_nx == 64) launch_xgreen_c<T, 64>(L, st, mode);
+        else launch_xgreen_c<T, 128>(L, st, mode);
         break;
     case 0: VLAUNCH((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
     case 1: VLAUNCH((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
@@ -3335,6 +3367,8 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 4>,
                          (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB>, (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 4>,
+                         (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB>, (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 2>,
+                         (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 4>,
                          (const void *)k_zfwd_s<T>, (const void *)k_y_s<T, false>, (const void *)k_y_s<T, true>,
                          (const void *)k_zinv_s<T>};
     for (const void *f : fns) {
@@ -3393,7 +3427,7 @@ static const size_t LDS_MAX = 160 * 1024;
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
     if (xv == 3) return (nx == 16 || nx == 32) ? 4 : (nx == 64 ? 2 : 0);
-    if (xv == 4) return (nx == 16 || nx == 32 || nx == 64) ? 8 : 0;   // FP64: a whole column block
+    if (xv == 4) return (nx == 16 || nx == 32 || nx == 64 || nx == 128) ? 8 : 0;   // FP64: a whole column block
     const int fit = X_MAXEL[xv] / (6 * nx);
     if (fit < 1) return 0;   // the variant cannot hold one column
     int tk = 1;
@@ -3443,11 +3477,12 @@ extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
         return (size_t)tk * 6 * nx * cb;
     }
     if (xv == 4) {
-        // x pass staged through LDS: Nx 16, 32 or 64; tk = a whole column
-        // block (8 FP64 / 16 FP32 columns), a half or a quarter of one (whole
-        // columns per wave: 1 at Nx = 64, else 2); six lines of Nx per column
-        const int nx = r->nx, tb = r->fft_f32 ? 16 : 8, cpw = nx == 64 ? 1 : 2;
-        if (nx != 16 && nx != 32 && nx != 64) return 0;
+        // x pass staged through LDS: Nx 16, 32, 64 or 128; tk = a whole
+        // column block (8 FP64 / 16 FP32 columns), a half or a quarter of one
+        // (whole columns per wave: 1 from Nx = 64 on, else 2); six lines of
+        // Nx per column
+        const int nx = r->nx, tb = r->fft_f32 ? 16 : 8, cpw = nx >= 64 ? 1 : 2;
+        if (nx != 16 && nx != 32 && nx != 64 && nx != 128) return 0;
         if ((tk != tb && 2 * tk != tb && 4 * tk != tb) || tk % cpw) return 0;
         return (size_t)tk * 6 * nx * cb;
     }
