@@ -418,6 +418,12 @@ __device__ void fft_lines(C *s, int nlines, int N, int lstride, int estride, con
 
 template <int NT> __device__ __forceinline__ void finalize_block(const RveDev &r);
 
+// Local block index lb of a forward plane launch: its finaliser block
+// (lb == 6 nx) is the launch's first block and the plane blocks follow, so
+// the finaliser's row sums overlap them; dispatched last, they added their
+// whole duration to the launch (22 us of row sums at 128^3).
+__device__ __forceinline__ int fin_first(int lb0, int nx) { return lb0 == 0 ? 6 * nx : lb0 - 1; }
+
 // The macro state is written only by single-thread finalisers in other
 // launches (or, racing harmlessly, by the finaliser block of a forward plane
 // launch, whose other blocks only write scratch spectra), so kernels read it
@@ -444,7 +450,8 @@ template <typename T, int NT, int MAXEL>
 __global__ __launch_bounds__(NT) void k_plane_fwd(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     using C = typename cplx_of<T>::t;
-    RVE_PROLOGUE(lb);
+    RVE_PROLOGUE(lb0);
+    const int lb = fin_first(lb0, r.nx);
     if (lb == 6 * r.nx) {   // the previous iteration's finaliser
         finalize_block<NT>(r);
         return;
@@ -653,7 +660,8 @@ template <typename T>
 __global__ __launch_bounds__(NT_SP) void k_zfwd_s(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
 {
     using C = typename cplx_of<T>::t;
-    RVE_PROLOGUE(lb);
+    RVE_PROLOGUE(lb0);
+    const int lb = fin_first(lb0, r.nx);
     if (lb == 6 * r.nx) {
         if (blockIdx.y == 0) finalize_block<NT_SP>(r);
         return;
@@ -924,7 +932,8 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__
     using C = typename cplx_of<T>::t;
     constexpr int L = P2Plan<N>::L, R = P2Plan<N>::R, NT = P2Plan<N>::NT;
     constexpr int NZH = N / 2 + 1, RS = 2 * NZH, KT = N / (2 * L);
-    RVE_PROLOGUE(lb);
+    RVE_PROLOGUE(lb0);
+    const int lb = fin_first(lb0, r.nx);
     if (lb == 6 * r.nx) {   // the previous iteration's finaliser
         finalize_block<NT>(r);
         return;
