@@ -22,11 +22,11 @@ extern "C" {
 /* Launch choices of the three FFT kernels for one grid size and precision:
  * plane-kernel variant (0: 64 threads / planes <= 32x32, 1: 256 / <= 64x64,
  * 2: 512 / <= 128x128, 3: 320 / <= 64x64, 4: line-parallel, square planes
- * of 16, 32, 64 or 128 only, 5: split z and y passes, any plane up to
+ * of 16, 32, 48, 64 or 128 only, 5: split z and y passes, any plane up to
  * 512x512), x-pass variant (0: 256 threads,
  * 1: 128, 2: 64, 3: line-parallel, one wave per column (tile 2 or 4 waves),
  * Nx = 16, 32 or 64 only, 4: staged through LDS, a block per whole, half or
- * quarter 128-byte column block, Nx = 16, 32, 64 or 128 only),
+ * quarter 128-byte column block, Nx = 16, 32, 48, 64 or 128 only),
  * x-pass tile (columns per block), and whether radix-8 stages are used in
  * the plane and x passes (variants 3 and 4 have fixed radices).  us_* are
  * the measured launch times (microseconds) of the chosen plane pair

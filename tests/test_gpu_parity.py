@@ -205,14 +205,17 @@ def test_mixed_precision_within_1e4(tmp_path):
                                          ((32, 32, 32), {"plv": 4, "xv": 4, "tk": 8}),
                                          ((2, 128, 128), {"plv": 4}), ((3, 128, 128), {"plv": 4}),
                                          ((128, 12, 16), {"xv": 4, "tk": 8}), ((128, 12, 16), {"xv": 4, "tk": 2}),
-                                         ((128, 8, 8), {"xv": 4, "tk": 4})])
+                                         ((128, 8, 8), {"xv": 4, "tk": 4}),
+                                         ((4, 48, 48), {"plv": 4}), ((48, 8, 12), {"xv": 4, "tk": 8}),
+                                         ((48, 12, 8), {"xv": 4, "tk": 2}), ((48, 8, 8), {"xv": 4, "tk": 4})])
 def test_fp64_parity_line_parallel(tmp_path, dims, tuning):
     # plane variant 4 (line-parallel kernels of square power-of-two planes,
     # any Nx; 128 x 128 planes take 16-point lines per lane), x-pass
     # variant 3 (line-parallel x pass, Nx = 16, 32, 64, any plane) and
     # x-pass variant 4 (staged through LDS, a whole, half or quarter column
-    # block per block, Nx = 16 to 128, two k points per thread at 128;
-    # 12 x 9 = 108 columns leave a partial last block), against the oracle
+    # block per block, Nx = 16 to 128, two k points per thread at 128, one
+    # or two at 48; 12 x 9 = 108 and 12 x 5 = 60 columns leave a partial
+    # last block), 48-point lines as 4 lanes of 12 points, against the oracle
     c = inputs.Case(dims=dims, n_grains=6, seed=19, n_steps=3)
     compare(tmp_path, c, 3, tuning=tuning)
 
@@ -231,16 +234,19 @@ def test_mixed_line_parallel_planes_against_oracle(tmp_path, tuning):
     assert rel(gf["sig"], of["sig"]) < 1e-6
 
 
-@pytest.mark.parametrize("tk", [16, 4])
-def test_mixed_staged_x_pass_nx128_against_oracle(tmp_path, tk):
+@pytest.mark.parametrize("dims,tuning", [((128, 8, 16), {"xv": 4, "tk": 16}), ((128, 8, 16), {"xv": 4, "tk": 4}),
+                                         ((48, 12, 8), {"xv": 4, "tk": 16}), ((48, 12, 8), {"xv": 4, "tk": 8}),
+                                         ((4, 48, 48), {"plv": 4})])
+def test_mixed_staged_x_pass_nx128_nx48_against_oracle(tmp_path, dims, tuning):
     # FP32 x pass staged through LDS at Nx = 128 (two k points per thread)
-    c = inputs.Case(dims=(128, 8, 16), n_grains=6, seed=37, n_steps=2)
+    # and Nx = 48 (one or two), FP32 line-parallel 48 x 48 planes
+    c = inputs.Case(dims=dims, n_grains=6, seed=37, n_steps=2)
     path, base = write(tmp_path, c)
-    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning={"xv": 4, "tk": tk})
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning=tuning)
     o, of = oracle_run(path, base, 2, fields=("sig",))
     for k in range(2):
         r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
-        say("incr", k + 1, "mixed (x-pass variant 4, Nx 128, tile", tk, ") vs oracle", r)
+        say("incr", k + 1, "mixed", dims, tuning, "vs oracle", r)
         assert r < 1e-4
     assert rel(gf["sig"], of["sig"]) < 1e-6
 

@@ -220,6 +220,41 @@ template <bool INV, typename C> struct Bfly<16, INV, C> {
     }
 };
 
+template <bool INV, typename C> struct Bfly<12, INV, C> {
+    __device__ __forceinline__ static void run(C *v)
+    {
+        using T = decltype(v[0].x);
+        // n = n1 + 4 n2, k = 3 k1 + k2: radix-3 over n2 for each n1, twiddle
+        // W12^(n1 k2), radix-4 over n1 for each k2
+        // (W12^j = c[j] - i s[j] forward, c[j] = cos(pi j / 6))
+        const T c[7] = {(T)1, (T)0.86602540378443865, (T)0.5, (T)0, (T)-0.5, (T)-0.86602540378443865, (T)-1};
+        const T s[7] = {(T)0, (T)0.5, (T)0.86602540378443865, (T)1, (T)0.86602540378443865, (T)0.5, (T)0};
+        C y[3][4];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) {
+            C t[3] = {v[n1], v[n1 + 4], v[n1 + 8]};
+            Bfly<3, INV, C>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) y[k2][n1] = t[k2];
+        }
+#pragma unroll
+        for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+            for (int k2 = 1; k2 < 3; ++k2) {
+                const int j = n1 * k2;
+                if (j == 3) y[k2][n1] = cmul_mi<INV>(y[k2][n1]);
+                else if (j == 6) y[k2][n1] = C{-y[k2][n1].x, -y[k2][n1].y};
+                else y[k2][n1] = cmul(y[k2][n1], C{c[j], INV ? s[j] : -s[j]});
+            }
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) {
+            Bfly<4, INV, C>::run(y[k2]);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) v[3 * k1 + k2] = y[k2][k1];
+        }
+    }
+};
+
 // Odd-prime butterflies (R = 5, 7) in the symmetric form: with
 // t_j = v_j + v_(R-j), u_j = v_j - v_(R-j) (j = 1..(R-1)/2),
 // X_0 = v_0 + sum t_j, and for k = 1..(R-1)/2
@@ -861,8 +896,8 @@ __global__ __launch_bounds__(NT_SP) void k_zinv_s(const RveDev *__restrict__ rta
 }
 
 // ------------------------------------------------------------------
-// Line-parallel plane kernels for square power-of-two planes (Ny = Nz = N,
-// N = 16, 32, 64, 128): every line of N points (a packed z-row pair, then a y
+// Line-parallel plane kernels for square planes (Ny = Nz = N, N = 16, 32,
+// 48, 64, 128): every line of N points (a packed z-row pair, then a y
 // column of the half spectrum) belongs to a group of L lanes of one wave,
 // R = N / L points per lane.  A line is transformed by an R-point DFT in
 // registers, the twiddles W_N^(n1 k1), one exchange through the line's own
@@ -875,6 +910,7 @@ __global__ __launch_bounds__(NT_SP) void k_zinv_s(const RveDev *__restrict__ rta
 template <int N> struct P2Plan;
 template <> struct P2Plan<16> { static constexpr int L = 4, R = 4, NT = 64; };
 template <> struct P2Plan<32> { static constexpr int L = 4, R = 8, NT = 128; };
+template <> struct P2Plan<48> { static constexpr int L = 4, R = 12, NT = 128; };
 template <> struct P2Plan<64> { static constexpr int L = 8, R = 8, NT = 320; };
 template <> struct P2Plan<128> { static constexpr int L = 8, R = 16, NT = 576; };
 
@@ -985,7 +1021,7 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_fwd_p2(const RveDev *__
         for (int t = 0; t <= KT; ++t) {
             const int k = n1 + L * t;
             if (k <= N / 2) {
-                const C z0 = line[k], z1 = line[(N - k) & (N - 1)];
+                const C z0 = line[k], z1 = line[(N - k) % N];
                 A[t] = C{(T)0.5 * (z0.x + z1.x), (T)0.5 * (z0.y - z1.y)};
                 B[t] = C{(T)0.5 * (z0.y + z1.y), (T)0.5 * (z1.x - z0.x)};
             }
@@ -1546,7 +1582,7 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 }
 
 // ------------------------------------------------------------------
-// x pass staged through LDS (x-pass variant 4, Nx = 16, 32, 64 or 128).  A block
+// x pass staged through LDS (x-pass variant 4, Nx = 16, 32, 48, 64 or 128).  A block
 // owns TK columns of one 128-byte column block (all of its TB columns, a
 // half or a quarter).  For each component the block's elements (kx, column)
 // form one contiguous run of Nx * TK complex values, so the spectrum and
@@ -1555,8 +1591,9 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 // column.  Phases, separated by block barriers: load -> forward line DFTs
 // (waves as in variant 3: groups of L lanes per line) -> Green operator per
 // k point -> inverse line DFTs -> store.  A wave holds one column's six
-// lines from Nx = 64 on, so at Nx = 128 a thread takes two k points,
-// (kx, j) and (kx + 64, j).
+// lines from Nx = 64 on (two columns' below), so at Nx = 128 a thread takes
+// two k points, (kx, j) and (kx + 64, j), and at Nx = 48 one or two,
+// (kx, j) and, for kx < 16, (kx + 32, j).
 //
 // In LDS, element (c, i, j) (component c, line index i, column j of the
 // block) has the row-major index a = (c Nx + i) TK + j, moved within its
@@ -1574,8 +1611,8 @@ template <typename T, int NX, int TK> struct XC {
     static constexpr int L = P2Plan<NX>::L, R = P2Plan<NX>::R;
     static constexpr int CPW = (NX >= 64) ? 1 : 2;                   // columns per wave (DFT role)
     static constexpr int WPB = TK / CPW, NT = 64 * WPB;
-    static constexpr int PPT = (NX > 64) ? NX / 64 : 1;               // k points per thread
-    static constexpr int KST = NT / TK;                              // kx step between them
+    static constexpr int KST = NT / TK;                              // kx step between a thread's k points
+    static constexpr int PPT = (NX + KST - 1) / KST;                 // k points per thread (at most)
     static_assert(TK == TB || 2 * TK == TB || 4 * TK == TB, "a block holds a column block, a half or a quarter");
     static_assert(TK % CPW == 0, "whole columns per wave");
     // the swizzle shifts s1 s2 s3 as hex digits (0: no term)
@@ -1619,11 +1656,13 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     // element (kx, col0 + j) of a component at base + kx * TB + j
     const long base = (long)(col0 / TB) * NX * TB + (col0 % TB);
     // point role: thread p takes the k points (pkx + h KST, column pj),
-    // h < PPT, at offsets pg + h KST TB of a component
+    // h < PPT, at offsets pg + h KST TB of a component (at Nx = 48 the
+    // second only for pkx < 16: hv(h))
     constexpr int PPT = X::PPT, KST = X::KST;
     const int p = (int)threadIdx.x, pkx = p / TK, pj = p % TK;
     const bool pt = (pkx < NX) && (col0 + pj < ncolt);
     const long pg = base + (long)pkx * TB + pj;
+    auto hv = [&](int h) { return NX % KST == 0 || pkx + h * KST < NX; };
     // DFT role: group g of L lanes of wave w holds component cc of column dj
     const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const int g = lane / L, n1 = lane % L;
@@ -1646,16 +1685,20 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     auto load_u = [&]() {
 #pragma unroll
         for (int h = 0; h < PPT; ++h)
+            if (hv(h)) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) u[h][i] = gload(usum + i * cstr + pg + h * KST * TB);
+                for (int i = 0; i < 3; ++i) u[h][i] = gload(usum + i * cstr + pg + h * KST * TB);
+            }
     };
     if (pt) {
         if (mode == 0) {
             C a[PPT][6];
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a[h][c] = gload(spec + c * cstr + pg + h * KST * TB);
+                    for (int c = 0; c < 6; ++c) a[h][c] = gload(spec + c * cstr + pg + h * KST * TB);
+                }
             // the running skew sums after the spectra (kept in that order: a
             // wait on a load also waits on the loads issued before it), and
             // on the first iteration too (then not used), since a load under
@@ -1666,18 +1709,24 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
             if (PPT == 1) load_u();
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
+                    for (int c = 0; c < 6; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
+                }
         } else {
             C a[PPT][3];
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) a[h][c] = gload(usum + c * cstr + pg + h * KST * TB);
+                    for (int c = 0; c < 3; ++c) a[h][c] = gload(usum + c * cstr + pg + h * KST * TB);
+                }
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
+                    for (int c = 0; c < 3; ++c) s[X::slot(c, pkx + h * KST, pj)] = a[h][c];
+                }
             // keeps this tail apart from the other branch's (a shared tail
             // would take this path's shorter wait counts)
             __asm__ volatile("" ::: "memory");
@@ -1698,7 +1747,7 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
         if (PPT > 1 && pt) load_u();
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < PPT; ++h) if (pt) {
+        for (int h = 0; h < PPT; ++h) if (pt && hv(h)) {
             const int kx = pkx + h * KST, col = col0 + pj;
             const int ky = col / nzh, kz = col - ky * nzh;
             C a[6];
@@ -1755,13 +1804,17 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
         if (mode == 0) {
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
+                    for (int c = 0; c < 6; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
+                }
         } else {
 #pragma unroll
             for (int h = 0; h < PPT; ++h)
+                if (hv(h)) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
+                    for (int c = 0; c < 3; ++c) gstore(spec + c * cstr + pg + h * KST * TB, s[X::slot(c, pkx + h * KST, pj)]);
+                }
         }
     }
 }
@@ -2028,7 +2081,21 @@ __device__ __forceinline__ void sum_partials(const double *part, int nb, bool du
 #pragma unroll
                 for (int k = 0; k < U; ++k) a += q[k];
             }
-            for (; b < nb; b += S) a += __builtin_nontemporal_load(gp + (long)b * 16 + e);
+            // the remaining rows of the stream (fewer than U) also go out
+            // together (a loop with a run-time trip count waits for each load
+            // before issuing the next), rows past the end read as the last
+            // row and add 0: the sum is the same, in the same order
+            if (b < nb) {
+                double q[U];
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    const int row = b + S * k;
+                    q[k] = __builtin_nontemporal_load(gp + (long)(row < nb ? row : nb - 1) * 16 + e);
+                }
+#pragma unroll
+                for (int k = 0; k < U; ++k)
+                    if (b + S * k < nb) a += q[k];
+            }
             red[l] = a;
         }
     }
@@ -3218,6 +3285,7 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
         const int n = L->p2_n;
         if (n == 16) VLAUNCH((k_plane_fwd_p2<T, 16>), dim3(L->n_pfwd), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         else if (n == 32) VLAUNCH((k_plane_fwd_p2<T, 32>), dim3(L->n_pfwd), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
+        else if (n == 48) VLAUNCH((k_plane_fwd_p2<T, 48>), dim3(L->n_pfwd), dim3(P2Plan<48>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         else if (n == 64) VLAUNCH((k_plane_fwd_p2<T, 64>), dim3(L->n_pfwd), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         else VLAUNCH((k_plane_fwd_p2<T, 128>), dim3(L->n_pfwd), dim3(P2Plan<128>::NT), L->pl_lds, st, L->tab, L->map_pfwd);
         break;
@@ -3252,6 +3320,7 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
         const int m = L->p2_n;
         if (m == 16) VLAUNCH((k_plane_inv_p2<T, 16>), dim3(n), dim3(P2Plan<16>::NT), L->pl_lds, st, L->tab, map, mode);
         else if (m == 32) VLAUNCH((k_plane_inv_p2<T, 32>), dim3(n), dim3(P2Plan<32>::NT), L->pl_lds, st, L->tab, map, mode);
+        else if (m == 48) VLAUNCH((k_plane_inv_p2<T, 48>), dim3(n), dim3(P2Plan<48>::NT), L->pl_lds, st, L->tab, map, mode);
         else if (m == 64) VLAUNCH((k_plane_inv_p2<T, 64>), dim3(n), dim3(P2Plan<64>::NT), L->pl_lds, st, L->tab, map, mode);
         else VLAUNCH((k_plane_inv_p2<T, 128>), dim3(n), dim3(P2Plan<128>::NT), L->pl_lds, st, L->tab, map, mode);
         break;
@@ -3305,6 +3374,7 @@ void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
     case 4:   // x pass staged through LDS (the grid's nx)
         if (L->p2_nx == 16) launch_xgreen_c<T, 16>(L, st, mode);
         else if (L->p2_nx == 32) launch_xgreen_c<T, 32>(L, st, mode);
+        else if (L->p2_nx == 48) launch_xgreen_c<T, 48>(L, st, mode);
         else if (L->p2_nx == 64) launch_xgreen_c<T, 64>(L, st, mode);
         else launch_xgreen_c<T, 128>(L, st, mode);
         break;
@@ -3365,6 +3435,7 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen<T, NT_XC, MAXEL_XC>,
                          (const void *)k_plane_fwd_p2<T, 16>, (const void *)k_plane_inv_p2<T, 16>,
                          (const void *)k_plane_fwd_p2<T, 32>, (const void *)k_plane_inv_p2<T, 32>,
+                         (const void *)k_plane_fwd_p2<T, 48>, (const void *)k_plane_inv_p2<T, 48>,
                          (const void *)k_plane_fwd_p2<T, 64>, (const void *)k_plane_inv_p2<T, 64>,
                          (const void *)k_plane_fwd_p2<T, 128>, (const void *)k_plane_inv_p2<T, 128>,
                          (const void *)k_xgreen_p2<T, 16, 2>, (const void *)k_xgreen_p2<T, 16, 4>,
@@ -3376,6 +3447,8 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen_c<T, 32, XC<T, 32, 8>::TB / 4>,
                          (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB>, (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 64, XC<T, 64, 8>::TB / 4>,
+                         (const void *)k_xgreen_c<T, 48, XC<T, 48, 8>::TB>, (const void *)k_xgreen_c<T, 48, XC<T, 48, 8>::TB / 2>,
+                         (const void *)k_xgreen_c<T, 48, XC<T, 48, 8>::TB / 4>,
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB>, (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 4>,
                          (const void *)k_zfwd_s<T>, (const void *)k_y_s<T, false>, (const void *)k_y_s<T, true>,
@@ -3436,7 +3509,7 @@ static const size_t LDS_MAX = 160 * 1024;
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
     if (xv == 3) return (nx == 16 || nx == 32) ? 4 : (nx == 64 ? 2 : 0);
-    if (xv == 4) return (nx == 16 || nx == 32 || nx == 64 || nx == 128) ? 8 : 0;   // FP64: a whole column block
+    if (xv == 4) return (nx == 16 || nx == 32 || nx == 48 || nx == 64 || nx == 128) ? 8 : 0;   // FP64: a whole column block
     const int fit = X_MAXEL[xv] / (6 * nx);
     if (fit < 1) return 0;   // the variant cannot hold one column
     int tk = 1;
@@ -3455,9 +3528,9 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
     const int el = r->ny * r->nzh;
     const size_t cb = r->fft_f32 ? 8 : 16;
     if (plv == 4) {
-        // line-parallel variant: square planes of 16, 32, 64 or 128, the plane only
+        // line-parallel variant: square planes of 16, 32, 48, 64 or 128, the plane only
         const int n = r->ny;
-        if (r->nz != n || (n != 16 && n != 32 && n != 64 && n != 128)) return 0;
+        if (r->nz != n || (n != 16 && n != 32 && n != 48 && n != 64 && n != 128)) return 0;
         const size_t b = (size_t)el * cb;
         return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
     }
@@ -3486,12 +3559,12 @@ extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
         return (size_t)tk * 6 * nx * cb;
     }
     if (xv == 4) {
-        // x pass staged through LDS: Nx 16, 32, 64 or 128; tk = a whole
+        // x pass staged through LDS: Nx 16, 32, 48, 64 or 128; tk = a whole
         // column block (8 FP64 / 16 FP32 columns), a half or a quarter of one
         // (whole columns per wave: 1 from Nx = 64 on, else 2); six lines of
         // Nx per column
         const int nx = r->nx, tb = r->fft_f32 ? 16 : 8, cpw = nx >= 64 ? 1 : 2;
-        if (nx != 16 && nx != 32 && nx != 64 && nx != 128) return 0;
+        if (nx != 16 && nx != 32 && nx != 48 && nx != 64 && nx != 128) return 0;
         if ((tk != tb && 2 * tk != tb && 4 * tk != tb) || tk % cpw) return 0;
         return (size_t)tk * 6 * nx * cb;
     }
