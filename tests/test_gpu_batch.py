@@ -198,3 +198,33 @@ def test_batch_member_with_split_plane_passes(tmp_path):
             assert rb[k][i]["iter"] == ro[k]["iter"]
             assert rel(rb[k][i]["sig_avg"], ro[k]["sig_avg"]) < 1e-8
         assert rel(fb[i]["sig"], fo["sig"]) < 1e-7
+
+
+def test_batch_members_48_point_lines(tmp_path):
+    """Members with 48 x 48 planes and Nx = 48 (the 48-point line-parallel
+    plane kernels and x pass staged through LDS, tuned per group) batched
+    with a 16^3 one: each equals its stand-alone run under the same launch
+    choices bit for bit, and matches the oracle."""
+    dims = [(4, 48, 48), (48, 8, 12), (16, 16, 16)]
+    specs = [inputs.RveSpec(d, 4, 1400 + i, np.diag([1e-3, -0.5e-3, -0.5e-3])) for i, d in enumerate(dims)]
+    cases = inputs.load_specs(str(tmp_path / "b"), specs)
+    n = 2
+    rb, fb, info = _run_batch(cases, n, "fp64", tune=True)
+    say("group-tuned choices", info["choices"])
+    assert info["choices"][0]["plv"] == 4
+    ra, fa = _run_alone(cases, n, "fp64", tune_choices=info["choices"])
+    for i, sp in enumerate(specs):
+        for k in range(n):
+            assert ra[i][k]["iter"] == rb[k][i]["iter"]
+            assert np.array_equal(ra[i][k]["sig_avg"], rb[k][i]["sig_avg"])
+        for f in FIELDS:
+            assert np.array_equal(fa[i][f], fb[i][f]), (i, f)
+        c = inputs.full_velocity_case(dims=sp.dims)
+        c.n_grains, c.seed = sp.n_grains, sp.seed
+        c.velgrad = (1e-3, -0.5e-3, -0.5e-3, 0.0, 0.0, 0.0)
+        path, base = write(tmp_path / ("o%d" % i), c)
+        ro, fo = oracle_run(path, base, n, fields=("sig",))
+        for k in range(n):
+            assert rb[k][i]["iter"] == ro[k]["iter"]
+            assert rel(rb[k][i]["sig_avg"], ro[k]["sig_avg"]) < 1e-8
+        assert rel(fb[i]["sig"], fo["sig"]) < 1e-7
