@@ -23,7 +23,8 @@ extern "C" {
  * plane-kernel variant (0: 64 threads / planes <= 32x32, 1: 256 / <= 64x64,
  * 2: 512 / <= 128x128, 3: 320 / <= 64x64, 4: line-parallel, square planes
  * of 16, 32, 48, 64 or 128 only, 5: split z and y passes, any plane up to
- * 512x512), x-pass variant (0: 256 threads,
+ * 512x512, 6: split passes with line-parallel transforms, 256x256 planes
+ * only), x-pass variant (0: 256 threads,
  * 1: 128, 2: 64, 3: line-parallel, one wave per column (tile 2 or 4 waves),
  * Nx = 16, 32 or 64 only, 4: staged through LDS, a block per whole, half or
  * quarter 128-byte column block, Nx = 16, 32, 48, 64 or 128 only),

@@ -77,9 +77,11 @@ def test_config3_128cube_mixed_against_fp64(tmp_path):
 def test_256cube_split_planes_mixed_against_fp64(tmp_path):
     """A 256^3 RVE (16.8 M voxels, 1000 grains, plane-strain full velocity
     control as in the 128^3 case): its planes are past one block's LDS, so
-    the split z and y passes run, and the x pass holds 256-point columns.
-    Too large for the oracle in a test; checked by MIXED against FP64 (1e-4
-    macro) and the size-independent properties the load fixes."""
+    the split z and y passes run (by default the line-parallel ones), and
+    the x pass holds 256-point columns.  Too large for the oracle in a
+    test; checked by MIXED against FP64 (1e-4 macro), by the line-parallel
+    split passes against the Stockham ones in FP64, and by the
+    size-independent properties the load fixes."""
     c = inputs.config_case(3)
     d = str(tmp_path)
     c.iter_max, c.n_steps = 20, 2
@@ -89,15 +91,25 @@ def test_256cube_split_planes_mixed_against_fp64(tmp_path):
     cs = voce_amd.load_case_voronoi(os.path.join(d, "input"), d, (256, 256, 256), 1000, 3)
     n = 2
     out = {}
-    for prec in ("fp64", "mixed"):
+    for key, prec, plv in (("fp64", "fp64", 6), ("mixed", "mixed", 6), ("fp64_v5", "fp64", 5)):
         s = voce_amd.Solver.from_case(cs, prec)
         t = s.tuning()
-        say(prec, "launch choice", t)
-        assert t["plv"] == 5
-        out[prec] = [s.increment() for _ in range(n)]
-        out[prec + "_sig"] = s.field("sig")
+        say(key, "launch choice", t)
+        if plv == 6:
+            assert t["plv"] == 6   # the default for 256 x 256 planes
+        else:
+            t["plv"] = 5
+            s.apply_tuning(t)
+        out[key] = [s.increment() for _ in range(n)]
+        out[key + "_sig"] = s.field("sig")
         s.close()
-        say(prec, [r["iter"] for r in out[prec]])
+        say(key, [r["iter"] for r in out[key]])
+    # the line-parallel split passes against the Stockham split passes, FP64
+    for k in range(n):
+        a, b = out["fp64"][k], out["fp64_v5"][k]
+        assert a["iter"] == b["iter"]
+        assert rel(a["sig_avg"], b["sig_avg"]) < 1e-10
+    assert rel(out["fp64_sig"], out["fp64_v5_sig"]) < 1e-9
     L = np.zeros((3, 3))
     L[0, 0], L[2, 2] = c.velgrad[0], c.velgrad[2]
     for k in range(n):

@@ -298,6 +298,27 @@ def test_fp64_parity_dimensions_above_128(tmp_path, dims, plv):
     compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "new_position"))
 
 
+@pytest.mark.parametrize("dims", [(2, 256, 256), (3, 256, 256)])
+def test_fp64_parity_line_parallel_split_passes(tmp_path, dims):
+    # plane variant 6 (split z and y launches whose 256-point lines are held
+    # by 16 lanes of 16 points), with the diagnostics' fluctuation sums,
+    # against the oracle
+    c = inputs.Case(dims=dims, n_grains=5, seed=43, n_steps=2)
+    compare(tmp_path, c, 2, fields=("sig", "eps", "crss", "tranmat", "disgrad", "new_position"), tuning={"plv": 6})
+
+
+def test_mixed_line_parallel_split_passes_against_oracle(tmp_path):
+    c = inputs.Case(dims=(2, 256, 256), n_grains=5, seed=47, n_steps=2)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning={"plv": 6})
+    o, of = oracle_run(path, base, 2, fields=("sig",))
+    for k in range(2):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed (line-parallel split passes) vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-5
+
+
 def test_mixed_split_plane_passes_against_oracle(tmp_path):
     c = inputs.Case(dims=(3, 256, 64), n_grains=5, seed=41, n_steps=2)
     path, base = write(tmp_path, c)

@@ -229,7 +229,7 @@ static int tune_group_timed(voce_batch *b, Group &g)
     voce_tune_choice best = keep, cand;
     double best_pl = 1e300, best_x = 1e300;
     for (int plv = 0; plv < VOCE_N_PLV && !rc; ++plv)
-        for (int r8 = 1; r8 >= (plv == 4 ? 1 : 0) && !rc; --r8) {   // plv 4 has fixed radices
+        for (int r8 = 1; r8 >= ((plv == 4 || plv == 6) ? 1 : 0) && !rc; --r8) {   // plv 4 and 6 have fixed radices
             cand = keep;
             cand.plv = plv;
             cand.pl_r8 = r8;

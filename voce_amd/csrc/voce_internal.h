@@ -177,8 +177,8 @@ int voce_launch_finalize(const LaunchSet *L, void *stream);   // pending iterati
 int voce_launch_poll(const RveDev *tab, const int *idx_dev, int n, int *flags_dev, void *stream);
 int voce_launch_init_fields(const RveDev *tab1, long V, const double *sig6_by_orient, void *stream);
 int voce_grid_supported(int n);
-#define VOCE_N_PLV 6                // plane-kernel variants (4: line-parallel square power-of-two planes,
-                                    // 5: split z and y passes)
+#define VOCE_N_PLV 7                // plane-kernel variants (4: line-parallel square planes,
+                                    // 5: split z and y passes, 6: line-parallel split passes, 256 x 256)
 // B-basis component a of the crystal-frame Schmid vector of system s of the
 // twelve {111}<110> systems in the order and signs of the reference's
 // Cu_p.sx.  System s = 3p + j is direction j of plane p; over the four planes

@@ -1196,6 +1196,251 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
     }
 }
 
+// ------------------------------------------------------------------
+// Line-parallel split plane passes (plane variant 6) for Ny = Nz = 256:
+// planes too large for one block's LDS, transformed as in variant 5 (the z
+// and y transforms of a plane are two launches, the half spectrum of z row y
+// at column y nzh + kz between them), but every 256-point line belongs to a
+// group of 16 lanes holding 16 points each (line_dft, one exchange through
+// the line's own LDS slots) instead of a block-wide Stockham transform with
+// a barrier per stage.  A block of SL_LPB groups owns SL_LPB row pairs (z
+// passes: blockIdx.y = chunk of row pairs), one of SLY<T>::LPB groups as
+// many kz columns (y pass).
+// The z passes have no block barrier: a group loads its row pair from HBM
+// and writes its half spectra (or E rows) straight back.  The y pass stages
+// its 256 x SLY<T>::LPB tile through LDS with whole 128-byte column runs per
+// load and store.
+// ------------------------------------------------------------------
+constexpr int SL_N = 256, SL_LPB = 8, SL_NT = 16 * SL_LPB;
+// y pass: kz columns per block and threads (FP64 16: 66 KB of LDS, two
+// blocks per CU, against one at 32; FP32 32)
+template <typename T> struct SLY { static constexpr int LPB = sizeof(T) == 8 ? 16 : 32, NT = 16 * LPB; };
+template <> struct P2Plan<256> { static constexpr int L = 16, R = 16, NT = SL_NT; };
+// dynamic LDS of the z passes (SL_LPB lines of 2 (N/2 + 1) slots) and of the
+// y pass (SLY<T>::LPB lines of N + 1)
+template <typename T> constexpr size_t sl_lds_z() { return (size_t)SL_LPB * (SL_N + 2) * sizeof(typename cplx_of<T>::t); }
+template <typename T> constexpr size_t sl_lds_y() { return (size_t)SLY<T>::LPB * (SL_N + 1) * sizeof(typename cplx_of<T>::t); }
+
+// forward z pass: rows 2p, 2p+1 of S6 as one complex row, z DFT, split into
+// half spectra, out to columns y nzh + k; the extra block lb == 6 nx
+// finalises the previous iteration
+template <typename T>
+__global__ __launch_bounds__(SL_NT) void k_zfwd_l(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int N = SL_N, L = P2Plan<N>::L, R = P2Plan<N>::R;
+    constexpr int NZH = N / 2 + 1, RS = 2 * NZH, KT = N / (2 * L);
+    RVE_PROLOGUE(lb0);
+    const int lb = fin_first(lb0, r.nx);
+    if (lb == 6 * r.nx) {
+        if (blockIdx.y == 0) finalize_block<SL_NT>(r);
+        return;
+    }
+    if (!iter_active(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx;
+    const int c = lb / nx, x = lb - c * nx;
+    const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
+    const int p = (int)blockIdx.y * SL_LPB + grp;   // row pair (N / 2 of them)
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_z;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
+    }
+    // the two rows in pairs of points (whole cache lines per group), into
+    // the line's slots in natural order
+    const double2 *src = (const double2 *)(r.S6 + (long)c * r.V + ((long)x * N + 2 * p) * N);
+    C *line = s + grp * RS;
+    double2 pa[R / 2], pb[R / 2];
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+        pa[q] = gload(src + n1 + L * q);
+        pb[q] = gload(src + N / 2 + n1 + L * q);
+    }
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+        const int z = 2 * (n1 + L * q);
+        line[z] = C{(T)pa[q].x, (T)pb[q].x};
+        line[z + 1] = C{(T)pa[q].y, (T)pb[q].y};
+    }
+    wave_lds_sync();
+    C v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = line[n1 + L * q];
+    line_dft<C, N, false>(v, tw, n1, [&](int i) { return line + i; });
+    wave_lds_sync();
+#pragma unroll
+    for (int o = 0; o < R; ++o) line[line_out_index<N>(n1, o)] = v[o];
+    wave_lds_sync();
+    // A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / 2i, k <= N/2
+    C *dst = (C *)r.spec + (long)c * r.nspec;
+#pragma unroll
+    for (int t = 0; t <= KT; ++t) {
+        const int k = n1 + L * t;
+        if (k <= N / 2) {
+            const C z0 = line[k], z1 = line[(N - k) % N];
+            gstore(dst + spec_at<C>(2 * p * NZH + k, x, nx), C{(T)0.5 * (z0.x + z1.x), (T)0.5 * (z0.y - z1.y)});
+            gstore(dst + spec_at<C>((2 * p + 1) * NZH + k, x, nx), C{(T)0.5 * (z0.y + z1.y), (T)0.5 * (z1.x - z0.x)});
+        }
+    }
+}
+
+// y pass, forward or inverse, in place on SLY<T>::LPB kz columns of one plane
+// (a row's columns of a tile straddle the 128-byte column blocks, nzh = 129
+// being odd: the wider the tile, the fewer blocks share a column block)
+template <typename T, bool INV>
+__global__ __launch_bounds__(SLY<T>::NT) void k_y_l(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int N = SL_N, L = P2Plan<N>::L, R = P2Plan<N>::R;
+    constexpr int NZH = N / 2 + 1, LS = N + 1;   // LDS line stride: column tiles without bank conflicts
+    RVE_PROLOGUE(lb);
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx;
+    const int c = lb / nx, x = lb - c * nx;
+    const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
+    const int k0 = (int)blockIdx.y * SLY<T>::LPB;
+    if (k0 >= NZH) return;
+    const int nk = min(SLY<T>::LPB, NZH - k0);
+    C *sp = (C *)r.spec + (long)c * r.nspec;
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_y;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
+    }
+    // element e = y SLY<T>::LPB + j: consecutive lanes on consecutive columns
+    constexpr int EPT = N * SLY<T>::LPB / SLY<T>::NT;
+    {
+        C a[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = (int)threadIdx.x + i * SLY<T>::NT, y = e / SLY<T>::LPB, j = e % SLY<T>::LPB;
+            if (j < nk) a[i] = gload(sp + spec_at<C>(y * NZH + k0 + j, x, nx));
+        }
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = (int)threadIdx.x + i * SLY<T>::NT, y = e / SLY<T>::LPB, j = e % SLY<T>::LPB;
+            if (j < nk) s[j * LS + y] = a[i];
+        }
+    }
+    __syncthreads();
+    if (grp < nk) {
+        C *col = s + grp * LS;
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = col[n1 + L * q];
+        line_dft<C, N, INV>(v, tw, n1, [&](int i) { return col + i; });
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 0; o < R; ++o) col[line_out_index<N>(n1, o)] = v[o];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const int e = (int)threadIdx.x + i * SLY<T>::NT, y = e / SLY<T>::LPB, j = e % SLY<T>::LPB;
+        if (j < nk) gstore(sp + spec_at<C>(y * NZH + k0 + j, x, nx), s[j * LS + y]);
+    }
+}
+
+// inverse z pass: half spectra of rows 2p (A) and 2p+1 (B), Hermitian
+// extended into Z = A + iB, inverse z DFT; mode 0: E6[c] += shift[c] +
+// result (creep: E6[c] = creep[c] + result); mode 1: WF[c] = result; the
+// fluctuation sums as in k_plane_inv_p2
+template <typename T>
+__global__ __launch_bounds__(SL_NT) void k_zinv_l(const RveDev *__restrict__ rtab, const int2 *__restrict__ bmap, int mode)
+{
+    using C = typename cplx_of<T>::t;
+    constexpr int N = SL_N, L = P2Plan<N>::L, R = P2Plan<N>::R;
+    constexpr int NZH = N / 2 + 1;
+    static_assert(L == R, "lane n1 holds z = n1 + L o after line_dft");
+    RVE_PROLOGUE(lb);
+    if (mode == 0 ? !iter_active(r.ms) : !rve_need_skew(r.ms)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C *s = (C *)smem;
+    const int nx = r.nx;
+    const int c = lb / nx, x = lb - c * nx;
+    const int grp = (int)threadIdx.x / L, n1 = (int)threadIdx.x % L;
+    const int p = (int)blockIdx.y * SL_LPB + grp;
+    const cptr<MacroState> ms = as_const(r.ms);
+    const bool creep_set = mode == 0 && (ms->creep != 0) && (ms->iter == 0);
+    C tw[R];
+    {
+        const C *gtw = (const C *)r.tw_z;
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) tw[k1] = gload(gtw + n1 * k1);
+    }
+    const C *sp = (const C *)r.spec + (long)c * r.nspec;
+    C v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int k = n1 + L * q, kk = (k <= N / 2) ? k : N - k;
+        C a = gload(sp + spec_at<C>(2 * p * NZH + kk, x, nx));
+        C b = gload(sp + spec_at<C>((2 * p + 1) * NZH + kk, x, nx));
+        if (k == 0 || 2 * k == N) { a.y = (T)0; b.y = (T)0; }
+        if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+        v[q] = C{a.x - b.y, a.y + b.x};
+    }
+    C *line = s + grp * N;
+    line_dft<C, N, true>(v, tw, n1, [&](int i) { return line + i; });
+    // lane n1 holds z = n1 + L o in v[o]: real part row 2p, imaginary part row 2p+1
+    const long off = (long)c * r.V + ((long)x * N + 2 * p) * N;
+    if (mode == 0) {
+        const gmut<double> E = as_global_mut(r.E6 + off);
+        const double sh = creep_set ? 0.0 : ms->shift6[c];
+        const double cr = ms->creep_sym6[c];
+        double ea[R], eb[R];
+        // E is loaded after the DFT, whose 16-point registers the loads
+        // would otherwise hold
+        __asm__ volatile("" ::: "memory");
+        if (!creep_set) {
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                ea[o] = E[n1 + L * o];
+                eb[o] = E[N + n1 + L * o];
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            const int z = n1 + L * o;
+            const double va = (double)v[o].x, vb = (double)v[o].y;
+            E[z] = creep_set ? cr + (sh + va) : ea[o] + (sh + va);
+            E[N + z] = creep_set ? cr + (sh + vb) : eb[o] + (sh + vb);
+        }
+        const int fs = (c == 1) ? 0 : (c == 2) ? 1 : (c == 3) ? 2 : (c == 5) ? 3 : -1;
+        if (r.FLU && fs >= 0) {
+            const gmut<double> F = as_global_mut(r.FLU + (long)fs * r.V + ((long)x * N + 2 * p) * N);
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                const int z = n1 + L * o;
+                F[z] += (double)v[o].x;
+                F[N + z] += (double)v[o].y;
+            }
+        }
+    } else {
+        const gmut<double> W = as_global_mut(r.WF + off);
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            const int z = n1 + L * o;
+            W[z] = (double)v[o].x;
+            W[N + z] = (double)v[o].y;
+        }
+        if (r.FLU && c < 2) {
+            const gmut<double> F = as_global_mut(r.FLU + (long)(4 + c) * r.V + ((long)x * N + 2 * p) * N);
+#pragma unroll
+            for (int o = 0; o < R; ++o) {
+                const int z = n1 + L * o;
+                F[z] += (double)v[o].x;
+                F[N + z] += (double)v[o].y;
+            }
+        }
+    }
+}
+
 // unit frequency vector, init.c:660-677
 template <typename T>
 __device__ __forceinline__ void kvec(int kx, int ky, int kz, int nx, int ny, int nz, T n[3])
@@ -2550,12 +2795,12 @@ __device__ __forceinline__ int newton_crystal(double sig[6], double ed[5], doubl
         // sigma afresh every step: a running update of K_c sigma would
         // carry the rounding of the large early steps (where dt edot and
         // dt D x nearly cancel) into the converged stress.
-        // One triangle of K_c,x (21 loads, issued together) serves the
-        // residual and the factorisation.
         // The hydrostatic component decouples (cubic elasticity: K_c has no
         // coupling beyond rounding, D has no hydrostatic row), so its Newton
         // step lands on h6 = -w6 / K_c66 and the rest is a 5x5 system in the
-        // deviatoric block of K_c,x (15 loads, issued together).
+        // deviatoric block of K_c,x: one triangle of it (15 loads, issued
+        // together) serves the residual and the factorisation.  (Loaded at
+        // the top of the step, before the flow rule, it spills registers.)
         double K[15];
 #pragma unroll
         for (int q = 0; q < 15; ++q) K[q] = kcv[q * 64];
@@ -3303,6 +3548,19 @@ void launch_plane_fwd(const LaunchSet *L, hipStream_t st)
         }
         break;
     }
+    case 6: {   // line-parallel split passes (256 x 256 planes)
+        const dim3 gz(L->n_pfwd, SL_N / 2 / SL_LPB), gy(L->n_plane6, (SL_N / 2 + SLY<T>::LPB) / SLY<T>::LPB);
+        const size_t zl = sl_lds_z<T>(), yl = sl_lds_y<T>();
+        if (g_kt0) {
+            hipExtLaunchKernelGGL((k_zfwd_l<T>), gz, dim3(SL_NT), zl, st, g_kt0, nullptr, 0, L->tab, L->map_pfwd);
+            hipExtLaunchKernelGGL((k_y_l<T, false>), gy, dim3(SLY<T>::NT), yl, st, nullptr, g_kt1, 0, L->tab,
+                                  L->map_plane6, 0);
+        } else {
+            hipLaunchKernelGGL((k_zfwd_l<T>), gz, dim3(SL_NT), zl, st, L->tab, L->map_pfwd);
+            hipLaunchKernelGGL((k_y_l<T, false>), gy, dim3(SLY<T>::NT), yl, st, L->tab, L->map_plane6, 0);
+        }
+        break;
+    }
     default: VLAUNCH((k_plane_fwd<T, NT_PL_L, MAXEL_PL_L>), dim3(L->n_pfwd), dim3(NT_PL_L), L->pl_lds, st, L->tab, L->map_pfwd); break;
     }
 }
@@ -3333,6 +3591,18 @@ void launch_plane_inv(const LaunchSet *L, hipStream_t st, int mode)
         } else {
             hipLaunchKernelGGL((k_y_s<T, true>), gy, dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
             hipLaunchKernelGGL((k_zinv_s<T>), gz, dim3(NT_SP), L->pl_lds, st, L->tab, map, mode);
+        }
+        break;
+    }
+    case 6: {   // line-parallel split passes (256 x 256 planes)
+        const dim3 gy(n, (SL_N / 2 + SLY<T>::LPB) / SLY<T>::LPB), gz(n, SL_N / 2 / SL_LPB);
+        const size_t zl = sl_lds_z<T>(), yl = sl_lds_y<T>();
+        if (g_kt0) {
+            hipExtLaunchKernelGGL((k_y_l<T, true>), gy, dim3(SLY<T>::NT), yl, st, g_kt0, nullptr, 0, L->tab, map, mode);
+            hipExtLaunchKernelGGL((k_zinv_l<T>), gz, dim3(SL_NT), zl, st, nullptr, g_kt1, 0, L->tab, map, mode);
+        } else {
+            hipLaunchKernelGGL((k_y_l<T, true>), gy, dim3(SLY<T>::NT), yl, st, L->tab, map, mode);
+            hipLaunchKernelGGL((k_zinv_l<T>), gz, dim3(SL_NT), zl, st, L->tab, map, mode);
         }
         break;
     }
@@ -3452,7 +3722,8 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB>, (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 4>,
                          (const void *)k_zfwd_s<T>, (const void *)k_y_s<T, false>, (const void *)k_y_s<T, true>,
-                         (const void *)k_zinv_s<T>};
+                         (const void *)k_zinv_s<T>, (const void *)k_zfwd_l<T>, (const void *)k_y_l<T, false>,
+                         (const void *)k_y_l<T, true>, (const void *)k_zinv_l<T>};
     for (const void *f : fns) {
         // dynamic LDS may use what the kernel's static LDS leaves of 160 KB
         hipFuncAttributes fa;
@@ -3490,12 +3761,14 @@ extern "C" int voce_vox_block(void) { return NT_VOX; }
 
 // Default plane variant of an RVE before tuning: the line-parallel kernels
 // where the grid allows them, else the 256- or 512-thread whole-plane
-// kernels, else the split passes; -1 if none fits.
+// kernels, else the split passes (line-parallel for 256 x 256 planes); -1 if
+// none fits.
 extern "C" int voce_default_plv(const RveDev *d)
 {
     if (voce_plane_lds(d, 4) > 0) return 4;
     if (voce_plane_lds(d, 1) > 0) return 1;
     if (voce_plane_lds(d, 2) > 0) return 2;
+    if (voce_plane_lds(d, 6) > 0) return 6;
     if (voce_plane_lds(d, 5) > 0) return 5;
     return -1;
 }
@@ -3532,6 +3805,13 @@ extern "C" size_t voce_plane_lds(const RveDev *r, int plv)
         const int n = r->ny;
         if (r->nz != n || (n != 16 && n != 32 && n != 48 && n != 64 && n != 128)) return 0;
         const size_t b = (size_t)el * cb;
+        return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
+    }
+    if (plv == 6) {
+        // line-parallel split passes: 256 x 256 planes (their launches
+        // take the kernels' own LDS sizes; this is the largest)
+        if (r->ny != SL_N || r->nz != SL_N) return 0;
+        const size_t b = r->fft_f32 ? std::max(sl_lds_z<float>(), sl_lds_y<float>()) : std::max(sl_lds_z<double>(), sl_lds_y<double>());
         return b + PLANE_STATIC_LDS <= LDS_MAX ? b : 0;
     }
     if (plv == 5) {

@@ -122,7 +122,7 @@ int measure(voce_ctx *c, voce_tune_choice *best)
     voce_tune_choice cand = keep, b = keep;
     // plane kernels: variant x radix set (x-pass choice held at its current value)
     for (int plv = 0; plv < VOCE_N_PLV && !rc; ++plv)
-        for (int r8 = 1; r8 >= (plv == 4 ? 1 : 0) && !rc; --r8) {   // plv 4 has fixed radices
+        for (int r8 = 1; r8 >= ((plv == 4 || plv == 6) ? 1 : 0) && !rc; --r8) {   // plv 4 and 6 have fixed radices
             cand = keep;
             cand.plv = plv;
             cand.pl_r8 = r8;
