@@ -27,7 +27,8 @@ extern "C" {
  * only), x-pass variant (0: 256 threads,
  * 1: 128, 2: 64, 3: line-parallel, one wave per column (tile 2 or 4 waves),
  * Nx = 16, 32 or 64 only, 4: staged through LDS, a block per whole, half or
- * quarter 128-byte column block, Nx = 16, 32, 48, 64 or 128 only),
+ * quarter 128-byte column block, Nx = 16, 32, 48, 64, 128 or 256 only, a
+ * half or a quarter at 256),
  * x-pass tile (columns per block), and whether radix-8 stages are used in
  * the plane and x passes (variants 3 and 4 have fixed radices).  us_* are
  * the measured launch times (microseconds) of the chosen plane pair

@@ -207,15 +207,17 @@ def test_mixed_precision_within_1e4(tmp_path):
                                          ((128, 12, 16), {"xv": 4, "tk": 8}), ((128, 12, 16), {"xv": 4, "tk": 2}),
                                          ((128, 8, 8), {"xv": 4, "tk": 4}),
                                          ((4, 48, 48), {"plv": 4}), ((48, 8, 12), {"xv": 4, "tk": 8}),
-                                         ((48, 12, 8), {"xv": 4, "tk": 2}), ((48, 8, 8), {"xv": 4, "tk": 4})])
+                                         ((48, 12, 8), {"xv": 4, "tk": 2}), ((48, 8, 8), {"xv": 4, "tk": 4}),
+                                         ((256, 8, 8), {"xv": 4, "tk": 4}), ((256, 6, 8), {"xv": 4, "tk": 2})])
 def test_fp64_parity_line_parallel(tmp_path, dims, tuning):
     # plane variant 4 (line-parallel kernels of square power-of-two planes,
     # any Nx; 128 x 128 planes take 16-point lines per lane), x-pass
     # variant 3 (line-parallel x pass, Nx = 16, 32, 64, any plane) and
     # x-pass variant 4 (staged through LDS, a whole, half or quarter column
-    # block per block, Nx = 16 to 128, two k points per thread at 128, one
-    # or two at 48; 12 x 9 = 108 and 12 x 5 = 60 columns leave a partial
-    # last block), 48-point lines as 4 lanes of 12 points, against the oracle
+    # block per block, Nx = 16 to 256, two k points per thread at 128, one
+    # or two at 48, up to three at 256 with four lines per wave; 12 x 9 =
+    # 108, 12 x 5 = 60 and 6 x 5 = 30 columns leave a partial last block),
+    # 48-point lines as 4 lanes of 12 points, against the oracle
     c = inputs.Case(dims=dims, n_grains=6, seed=19, n_steps=3)
     compare(tmp_path, c, 3, tuning=tuning)
 
@@ -236,7 +238,8 @@ def test_mixed_line_parallel_planes_against_oracle(tmp_path, tuning):
 
 @pytest.mark.parametrize("dims,tuning", [((128, 8, 16), {"xv": 4, "tk": 16}), ((128, 8, 16), {"xv": 4, "tk": 4}),
                                          ((48, 12, 8), {"xv": 4, "tk": 16}), ((48, 12, 8), {"xv": 4, "tk": 8}),
-                                         ((4, 48, 48), {"plv": 4})])
+                                         ((4, 48, 48), {"plv": 4}), ((256, 8, 12), {"xv": 4, "tk": 8}),
+                                         ((256, 8, 12), {"xv": 4, "tk": 4})])
 def test_mixed_staged_x_pass_nx128_nx48_against_oracle(tmp_path, dims, tuning):
     # FP32 x pass staged through LDS at Nx = 128 (two k points per thread)
     # and Nx = 48 (one or two), FP32 line-parallel 48 x 48 planes

@@ -192,7 +192,7 @@ static inline double voce_fcc_schmid(int s, int a)
     const double unit = (a == 1) ? 0.5 : 0.5 / sqrt(3.0);
     return ((w == 1 || w == 2) ? -1.0 : 1.0) * K[a][j] * unit;
 }
-#define VOCE_N_XV 5                 // x-pass variants (3: line-parallel, Nx 16, 32, 64; 4: staged through LDS, Nx 16, 32, 48, 64, 128)
+#define VOCE_N_XV 5                 // x-pass variants (3: line-parallel, Nx 16, 32, 64; 4: staged through LDS, Nx 16, 32, 48, 64, 128, 256)
 int voce_kernels_prepare(void);
 int voce_xgreen_tile(int nx, int xv);
 int voce_vox_block(void);   // threads per block of the per-voxel kernels

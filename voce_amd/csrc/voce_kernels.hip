@@ -1827,7 +1827,8 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 }
 
 // ------------------------------------------------------------------
-// x pass staged through LDS (x-pass variant 4, Nx = 16, 32, 48, 64 or 128).  A block
+// x pass staged through LDS (x-pass variant 4, Nx = 16, 32, 48, 64, 128 or
+// 256).  A block
 // owns TK columns of one 128-byte column block (all of its TB columns, a
 // half or a quarter).  For each component the block's elements (kx, column)
 // form one contiguous run of Nx * TK complex values, so the spectrum and
@@ -1836,9 +1837,10 @@ __global__ __launch_bounds__(64 * WPB) void k_xgreen_p2(const RveDev *__restrict
 // column.  Phases, separated by block barriers: load -> forward line DFTs
 // (waves as in variant 3: groups of L lanes per line) -> Green operator per
 // k point -> inverse line DFTs -> store.  A wave holds one column's six
-// lines from Nx = 64 on (two columns' below), so at Nx = 128 a thread takes
-// two k points, (kx, j) and (kx + 64, j), and at Nx = 48 one or two,
-// (kx, j) and, for kx < 16, (kx + 32, j).
+// lines from Nx = 64 on (two columns' below, four lines of the block's 6 TK
+// at 256, whose 256-point lines take 16 lanes), so at Nx = 128 a thread takes
+// two k points, (kx, j) and (kx + 64, j), at Nx = 48 one or two, (kx, j)
+// and, for kx < 16, (kx + 32, j), and at 256 up to three, 96 apart.
 //
 // In LDS, element (c, i, j) (component c, line index i, column j of the
 // block) has the row-major index a = (c Nx + i) TK + j, moved within its
@@ -1854,12 +1856,15 @@ template <typename T, int NX, int TK> struct XC {
     using C = typename cplx_of<T>::t;
     static constexpr int TB = 128 / (int)sizeof(C);                 // columns per 128-byte block
     static constexpr int L = P2Plan<NX>::L, R = P2Plan<NX>::R;
-    static constexpr int CPW = (NX >= 64) ? 1 : 2;                   // columns per wave (DFT role)
-    static constexpr int WPB = TK / CPW, NT = 64 * WPB;
+    // DFT role: a wave holds LPW of the block's 6 TK lines (whole columns'
+    // six lines when its 64 / L groups allow, else 64 / L lines)
+    static constexpr int GPW = 64 / L, LPW = GPW >= 6 ? (GPW / 6) * 6 : GPW;
+    static constexpr int WPB = (6 * TK) / LPW, NT = 64 * WPB;
     static constexpr int KST = NT / TK;                              // kx step between a thread's k points
     static constexpr int PPT = (NX + KST - 1) / KST;                 // k points per thread (at most)
     static_assert(TK == TB || 2 * TK == TB || 4 * TK == TB, "a block holds a column block, a half or a quarter");
-    static_assert(TK % CPW == 0, "whole columns per wave");
+    static_assert((6 * TK) % LPW == 0, "whole waves of lines");
+    static_assert(NT <= 1024, "threads per block");
     // the swizzle shifts s1 s2 s3 as hex digits (0: no term)
     static constexpr int SW = sizeof(C) == 16
         ? (NX == 16 ? (TK == 4 ? 0x012 : 0x002) : NX == 32 ? 0x013 : (TK == 8 ? 0x003 : TK == 4 ? 0x002 : 0x001))
@@ -1878,7 +1883,7 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
 {
     using X = XC<T, NX, TK>;
     using C = typename X::C;
-    constexpr int TB = X::TB, L = X::L, R = X::R, CPW = X::CPW;
+    constexpr int TB = X::TB, L = X::L, R = X::R;
     RVE_PROLOGUE(lb);
     const MacroState *ms = r.ms;
     if (mode == 0 ? !iter_active(ms) : !rve_need_skew(ms)) return;
@@ -1908,11 +1913,15 @@ __global__ __launch_bounds__((XC<T, NX, TK>::NT)) void k_xgreen_c(const RveDev *
     const bool pt = (pkx < NX) && (col0 + pj < ncolt);
     const long pg = base + (long)pkx * TB + pj;
     auto hv = [&](int h) { return NX % KST == 0 || pkx + h * KST < NX; };
-    // DFT role: group g of L lanes of wave w holds component cc of column dj
+    // DFT role: group g of L lanes of wave w holds line li = w LPW + g,
+    // component cc of column dj
+    constexpr int LPW = X::LPW;
     const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const int g = lane / L, n1 = lane % L;
-    const int cc = g % 6, dj = w * CPW + g / 6;
-    const bool dft = (g < 6 * CPW) && (col0 + dj < ncolt) && (mode == 0 || cc < 3);
+    // (whole columns per wave: the same as cc = g % 6, dj = w LPW / 6 + g / 6)
+    const int li = w * LPW + g;
+    const int cc = LPW % 6 == 0 ? g % 6 : li % 6, dj = LPW % 6 == 0 ? w * (LPW / 6) + g / 6 : li / 6;
+    const bool dft = (g < LPW) && (col0 + dj < ncolt) && (mode == 0 || cc < 3);
     C *spec = (C *)r.spec;
     C *usum = (C *)r.usum;
     const bool first = (as_const(ms)->iter == 0);
@@ -3632,6 +3641,20 @@ void launch_xgreen_c(const LaunchSet *L, hipStream_t st, int mode)
                 mode);
 }
 
+// Nx = 256: half or quarter column blocks only (a whole one takes 1536
+// threads in FP32, 196 KB of LDS in FP64)
+template <typename T>
+void launch_xgreen_c256(const LaunchSet *L, hipStream_t st, int mode)
+{
+    constexpr int TB = 128 / (int)sizeof(typename cplx_of<T>::t);
+    if (L->p2_tk == TB / 2)
+        VLAUNCH((k_xgreen_c<T, 256, TB / 2>), dim3(L->n_x), dim3(XC<T, 256, TB / 2>::NT), L->x_lds, st, L->tab, L->map_x,
+                mode);
+    else
+        VLAUNCH((k_xgreen_c<T, 256, TB / 4>), dim3(L->n_x), dim3(XC<T, 256, TB / 4>::NT), L->x_lds, st, L->tab, L->map_x,
+                mode);
+}
+
 template <typename T>
 void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
 {
@@ -3646,7 +3669,8 @@ void launch_xgreen(const LaunchSet *L, hipStream_t st, int mode)
         else if (L->p2_nx == 32) launch_xgreen_c<T, 32>(L, st, mode);
         else if (L->p2_nx == 48) launch_xgreen_c<T, 48>(L, st, mode);
         else if (L->p2_nx == 64) launch_xgreen_c<T, 64>(L, st, mode);
-        else launch_xgreen_c<T, 128>(L, st, mode);
+        else if (L->p2_nx == 128) launch_xgreen_c<T, 128>(L, st, mode);
+        else launch_xgreen_c256<T>(L, st, mode);
         break;
     case 0: VLAUNCH((k_xgreen<T, NT_X, MAXEL_X>), dim3(L->n_x), dim3(NT_X), L->x_lds, st, L->tab, L->map_x, mode); break;
     case 1: VLAUNCH((k_xgreen<T, NT_XB, MAXEL_XB>), dim3(L->n_x), dim3(NT_XB), L->x_lds, st, L->tab, L->map_x, mode); break;
@@ -3721,6 +3745,7 @@ hipError_t set_lds_limits()
                          (const void *)k_xgreen_c<T, 48, XC<T, 48, 8>::TB / 4>,
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB>, (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 2>,
                          (const void *)k_xgreen_c<T, 128, XC<T, 128, 8>::TB / 4>,
+                         (const void *)k_xgreen_c<T, 256, XC<T, 256, 8>::TB / 2>, (const void *)k_xgreen_c<T, 256, XC<T, 256, 8>::TB / 4>,
                          (const void *)k_zfwd_s<T>, (const void *)k_y_s<T, false>, (const void *)k_y_s<T, true>,
                          (const void *)k_zinv_s<T>, (const void *)k_zfwd_l<T>, (const void *)k_y_l<T, false>,
                          (const void *)k_y_l<T, true>, (const void *)k_zinv_l<T>};
@@ -3782,7 +3807,7 @@ static const size_t LDS_MAX = 160 * 1024;
 extern "C" int voce_xgreen_tile(int nx, int xv)
 {
     if (xv == 3) return (nx == 16 || nx == 32) ? 4 : (nx == 64 ? 2 : 0);
-    if (xv == 4) return (nx == 16 || nx == 32 || nx == 48 || nx == 64 || nx == 128) ? 8 : 0;   // FP64: a whole column block
+    if (xv == 4) return (nx == 16 || nx == 32 || nx == 48 || nx == 64 || nx == 128) ? 8 : (nx == 256 ? 4 : 0);   // FP64: a whole column block (a half at 256)
     const int fit = X_MAXEL[xv] / (6 * nx);
     if (fit < 1) return 0;   // the variant cannot hold one column
     int tk = 1;
@@ -3839,14 +3864,14 @@ extern "C" size_t voce_xgreen_lds(const RveDev *r, int xv, int tk)
         return (size_t)tk * 6 * nx * cb;
     }
     if (xv == 4) {
-        // x pass staged through LDS: Nx 16, 32, 48, 64 or 128; tk = a whole
-        // column block (8 FP64 / 16 FP32 columns), a half or a quarter of one
-        // (whole columns per wave: 1 from Nx = 64 on, else 2); six lines of
-        // Nx per column
+        // x pass staged through LDS: Nx 16, 32, 48, 64, 128 or 256; tk = a
+        // whole column block (8 FP64 / 16 FP32 columns), a half or a quarter
+        // of one (at 256 a half or a quarter); six lines of Nx per column
         const int nx = r->nx, tb = r->fft_f32 ? 16 : 8, cpw = nx >= 64 ? 1 : 2;
-        if (nx != 16 && nx != 32 && nx != 48 && nx != 64 && nx != 128) return 0;
-        if ((tk != tb && 2 * tk != tb && 4 * tk != tb) || tk % cpw) return 0;
-        return (size_t)tk * 6 * nx * cb;
+        if (nx != 16 && nx != 32 && nx != 48 && nx != 64 && nx != 128 && nx != 256) return 0;
+        if ((tk != tb && 2 * tk != tb && 4 * tk != tb) || tk % cpw || (nx == 256 && tk == tb)) return 0;
+        const size_t b = (size_t)tk * 6 * nx * cb;
+        return b <= LDS_MAX ? b : 0;
     }
     if (xv < 0 || xv > 2 || tk < 1 || (tk & (tk - 1)) || 6 * tk * r->nx > X_MAXEL[xv]) return 0;
     // 6 spectra + 3 skew sums, lines padded to nx + 1, then the twiddles
