@@ -170,9 +170,11 @@ def test_batch_radix5_radix7_odd_members(tmp_path):
 
 def test_batch_member_with_split_plane_passes(tmp_path):
     """A 3 x 256 x 64 member (planes past one block's LDS: the split z and y
-    passes) batched with a 16^3 one: each equals its stand-alone run bit for
-    bit and matches the oracle."""
-    specs = [inputs.RveSpec(d, 4, 1300 + i, np.zeros((3, 3))) for i, d in enumerate([(16, 16, 16), (3, 256, 64)])]
+    passes) and a 2 x 256 x 256 one (the line-parallel split passes) batched
+    with a 16^3 one: each equals its stand-alone run bit for bit and matches
+    the oracle."""
+    specs = [inputs.RveSpec(d, 4, 1300 + i, np.zeros((3, 3)))
+             for i, d in enumerate([(16, 16, 16), (3, 256, 64), (2, 256, 256)])]
     for sp in specs:
         sp.velgrad = np.diag([1e-3, -0.5e-3, -0.5e-3])
     cases = inputs.load_specs(str(tmp_path / "b"), specs)
@@ -180,7 +182,7 @@ def test_batch_member_with_split_plane_passes(tmp_path):
     ra, fa = _run_alone(cases, n, "fp64")
     rb, fb, info = _run_batch(cases, n, "fp64", tune=False)
     b = voce_amd.Batch(cases, "fp64", tune=False)
-    assert b.member(1).tuning()["plv"] == 5
+    assert b.member(1).tuning()["plv"] == 5 and b.member(2).tuning()["plv"] == 6
     b.close()
     for i, sp in enumerate(specs):
         for k in range(n):
