@@ -127,17 +127,19 @@ def cpu_baseline(path, base, sample_iters, threads):
     return V * r["iter"] / dt, dt, r["iter"]
 
 
-def run_plane_strain_128(d, steps, warmup, precisions=("mixed", "fp64")):
+def run_plane_strain_128(d, steps, warmup, precisions=("mixed", "fp64"), n=128):
     """128^3 / 1000-grain plane-strain compression (velocity gradient
     diag(1, 0, -1) 1e-3/s, dt 1 s): MIXED, and FP64 on the same seed for the
-    error of the macroscopic stress curve and of the final stress field."""
+    error of the macroscopic stress curve and of the final stress field;
+    n^3 voxels instead of 128^3 with n."""
     import voce_amd
     from voce_amd import inputs
     c = inputs.config_case(3)
+    c.dims = (n, n, n)
     c.n_steps = steps + warmup
-    d3 = os.path.join(d, "plane_strain_128")
+    d3 = os.path.join(d, "plane_strain_%d" % n)
     os.makedirs(d3, exist_ok=True)
-    log("generating 128^3 / 1000-grain Voronoi mesh in", d3)
+    log("generating %d^3 / 1000-grain Voronoi mesh in" % n, d3)
     inputs.write_case(d3, c)
     cs = voce_amd.load_case(os.path.join(d3, "input"), d3)
     res, curves, fields = {}, {}, {}
@@ -162,12 +164,12 @@ def run_plane_strain_128(d, steps, warmup, precisions=("mixed", "fp64")):
         res[prec] = {"voxel_iterations_per_s": round(V * it / wall, 1), "fft_iterations": it,
                      "ms_per_increment": round(wall / steps * 1e3, 3), "wall_s": round(wall, 3),
                      "fft_launch_choice": tuned}
-        log("128^3 %s: %d increments, %d iterations, %.3f s, %.4g voxel-it/s" % (prec, steps, it, wall, V * it / wall))
+        log("%d^3 %s: %d increments, %d iterations, %.3f s, %.4g voxel-it/s" % (n, prec, steps, it, wall, V * it / wall))
     cs.free()
-    out = {"workload": "single RVE, 128^3 voxels, 1000 Voronoi grains (seed 3), FCC Cu-like, plane-strain "
+    out = {"workload": "single RVE, %d^3 voxels, 1000 Voronoi grains (seed 3), FCC Cu-like, plane-strain "
                         "compression L = diag(1, 0, -1) 1e-3/s, dt 1 s, Err 1e-5, IterMax 100; timed: increments "
                         "1-%d (%.1f %% strain) of a fresh context after %d warm-up increments on another"
-                        % (steps, steps * 0.1, warmup),
+                        % (n, steps, steps * 0.1, warmup),
            "voxels": V, "increments": steps}
     out.update(res)
     if len(res) == 2:
@@ -321,8 +323,10 @@ def main():
     ap.add_argument("--no-serve-fp64", action="store_true")
     ap.add_argument("--ps128-steps", type=int, default=100)
     ap.add_argument("--no-ps128", action="store_true")
-    # only the 128^3 plane-strain leg, in the named precisions (profiling)
+    # only the 128^3 plane-strain leg, in the named precisions (profiling);
+    # --ps-n sets its grid (n^3) there
     ap.add_argument("--ps128-only", choices=("mixed", "fp64", "both"))
+    ap.add_argument("--ps-n", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the RVE lives whole on one MI355X and nothing here "
@@ -333,7 +337,8 @@ def main():
     d = tempfile.mkdtemp(prefix="voce_bench_")
     if args.ps128_only:
         precs = ("mixed", "fp64") if args.ps128_only == "both" else (args.ps128_only,)
-        print(json.dumps({"plane_strain_128": run_plane_strain_128(d, args.ps128_steps, 1, precs)}), flush=True)
+        print(json.dumps({"plane_strain_%d" % args.ps_n: run_plane_strain_128(d, args.ps128_steps, 1, precs, args.ps_n)}),
+              flush=True)
         return
     path, case = make_case(d, args.steps)
     nx, ny, nz = case.dims

@@ -1208,8 +1208,9 @@ __global__ __launch_bounds__(P2Plan<N>::NT) void k_plane_inv_p2(const RveDev *__
 // many kz columns (y pass).
 // The z passes have no block barrier: a group loads its row pair from HBM
 // and writes its half spectra (or E rows) straight back.  The y pass stages
-// its 256 x SLY<T>::LPB tile through LDS with whole 128-byte column runs per
-// load and store.
+// its 256 x SLY<T>::LPB tile through LDS, consecutive lanes on consecutive
+// columns of a row (a row's run of the tile spans two 128-byte column
+// blocks, since nzh = 129 is odd).
 // ------------------------------------------------------------------
 constexpr int SL_N = 256, SL_LPB = 8, SL_NT = 16 * SL_LPB;
 // y pass: kz columns per block and threads (FP64 16: 66 KB of LDS, two
