@@ -175,7 +175,7 @@ int voce_increment(voce_ctx *ctx, voce_incr_report *report);
  * voce_ctx_create has built).  Results are bit-identical to voce_increment.
  * Limits: a context of voce_ctx_create (not a batch member or server id)
  * with IterMax >= 1, whose workspace fits the array (one 64^3 RVE in FP64,
- * about 73 MiB; larger RVEs return VOCE_ERR_SCRATCH).  Every call on one
+ * about 61 MiB; larger RVEs return VOCE_ERR_SCRATCH).  Every call on one
  * device shares that device's array: calls that overlap on a device race. */
 #define VOCE_NOALLOC_MAX_BYTES (96u << 20)
 int voce_increment_noalloc(voce_ctx *ctx, voce_incr_report *report);
