@@ -127,7 +127,7 @@ def cpu_baseline(path, base, sample_iters, threads):
     return V * r["iter"] / dt, dt, r["iter"]
 
 
-def run_plane_strain_128(d, steps, warmup, precisions=("mixed", "fp64"), n=128):
+def run_plane_strain(d, steps, warmup, precisions=("mixed", "fp64"), n=128):
     """128^3 / 1000-grain plane-strain compression (velocity gradient
     diag(1, 0, -1) 1e-3/s, dt 1 s): MIXED, and FP64 on the same seed for the
     error of the macroscopic stress curve and of the final stress field;
@@ -337,7 +337,7 @@ def main():
     d = tempfile.mkdtemp(prefix="voce_bench_")
     if args.ps128_only:
         precs = ("mixed", "fp64") if args.ps128_only == "both" else (args.ps128_only,)
-        print(json.dumps({"plane_strain_%d" % args.ps_n: run_plane_strain_128(d, args.ps128_steps, 1, precs, args.ps_n)}),
+        print(json.dumps({"plane_strain_%d" % args.ps_n: run_plane_strain(d, args.ps128_steps, 1, precs, args.ps_n)}),
               flush=True)
         return
     path, case = make_case(d, args.steps)
@@ -431,7 +431,7 @@ def main():
         log("mixed: %.3f s, %.4g voxel-it/s, macro rel err %.2e" % (mwall, V * miters / mwall, err))
 
     if not args.no_ps128:
-        out["plane_strain_128"] = run_plane_strain_128(d, args.ps128_steps, 1)
+        out["plane_strain_128"] = run_plane_strain(d, args.ps128_steps, 1)
 
     rve_inc_fp64 = out["rve_increments_per_s"]
     if not args.no_batch:
