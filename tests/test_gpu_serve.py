@@ -325,14 +325,16 @@ def test_fp64_server_with_audits(tmp_path):
     assert len(aud) == 5 and max(aud) < 1e-10
 
 
-def test_served_split_plane_id_equals_context(tmp_path):
-    """An id whose planes take the split z and y passes (3 x 256 x 64) served
-    beside a 16^3 id: its increments equal the same increments on a
-    stand-alone context, and FP64 audits of it report the mixed error."""
-    specs = [inputs.RveSpec(d, 4, 7100 + i, np.zeros((3, 3))) for i, d in enumerate([(3, 256, 64), (16, 16, 16)])]
+@pytest.mark.parametrize("dims,plv", [((3, 256, 64), 5), ((2, 256, 256), 6)])
+def test_served_split_plane_id_equals_context(tmp_path, dims, plv):
+    """An id whose planes take the split z and y passes (3 x 256 x 64; 256 x
+    256 planes: the line-parallel ones) served beside a 16^3 id: its
+    increments equal the same increments on a stand-alone context, and FP64
+    audits of it report the mixed error."""
+    specs = [inputs.RveSpec(d, 4, 7100 + i, np.zeros((3, 3))) for i, d in enumerate([dims, (16, 16, 16)])]
     cases = inputs.load_specs(str(tmp_path), specs)
     s = _server(cases)
-    assert s.rve(0).tuning()["plv"] == 5
+    assert s.rve(0).tuning()["plv"] == plv
     rng = inputs.SplitMix64(29)
     reqs = []
     for t in range(8):
