@@ -276,6 +276,17 @@ def test_fp64_parity_radix5_radix7_odd_planes(tmp_path, dims, n):
     compare(tmp_path, c, n)
 
 
+@pytest.mark.parametrize("dims,tuning", [((48, 48, 48), {"xv": 4, "tk": 8}), ((24, 24, 24), None),
+                                         ((96, 32, 32), None), ((80, 16, 16), None)])
+def test_fp64_parity_intermediate_sizes(tmp_path, dims, tuning):
+    # whole grids between the workloads' sizes against the oracle: 48^3 with
+    # the 48-point line-parallel planes and x pass, 24^3 and 80 x 16 x 16
+    # (radix 3 and 5) on the Stockham kernels, 96 x 32 x 32 (a radix-3 x pass
+    # beside line-parallel 32-point planes)
+    c = inputs.Case(dims=dims, n_grains=12, seed=59, n_steps=2)
+    compare(tmp_path, c, 2, tuning=tuning)
+
+
 @pytest.mark.parametrize("dims", [(16, 16, 16), (12, 10, 14), (9, 7, 8), (8, 30, 20)])
 def test_fp64_parity_split_plane_passes_forced(tmp_path, dims):
     # plane variant 5 (separate z and y launches, each block walking its
