@@ -287,6 +287,19 @@ def test_fp64_parity_intermediate_sizes(tmp_path, dims, tuning):
     compare(tmp_path, c, 2, tuning=tuning)
 
 
+@pytest.mark.parametrize("dims,tuning", [((48, 48, 48), {"xv": 4, "tk": 16}), ((24, 24, 24), None)])
+def test_mixed_intermediate_sizes_against_oracle(tmp_path, dims, tuning):
+    c = inputs.Case(dims=dims, n_grains=12, seed=61, n_steps=2)
+    path, base = write(tmp_path, c)
+    g, gf = gpu_run(path, base, 2, "mixed", fields=("sig",), tuning=tuning)
+    o, of = oracle_run(path, base, 2, fields=("sig",))
+    for k in range(2):
+        r = rel(g[k]["sig_avg"], o[k]["sig_avg"])
+        say("incr", k + 1, "mixed", dims, "vs oracle", r)
+        assert r < 1e-4
+    assert rel(gf["sig"], of["sig"]) < 1e-6
+
+
 @pytest.mark.parametrize("dims", [(16, 16, 16), (12, 10, 14), (9, 7, 8), (8, 30, 20)])
 def test_fp64_parity_split_plane_passes_forced(tmp_path, dims):
     # plane variant 5 (separate z and y launches, each block walking its
