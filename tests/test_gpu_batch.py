@@ -114,6 +114,28 @@ def test_batch_matches_oracle_symmetric_L(tmp_path):
         assert rel(fb[i]["crss"], fo["crss"]) < 1e-9
 
 
+def test_mixed_batch_matches_oracle_symmetric_L(tmp_path):
+    """The MIXED batch (FP32 FFTs and Green operator per member) against the
+    FP64 oracle, W = 0: macro stress within 1e-4 at every increment."""
+    say("mixed batch vs oracle")
+    specs = _specs(skew=False)[:3]
+    cases = inputs.load_specs(str(tmp_path / "b"), specs)
+    n = 3
+    rb, fb, _ = _run_batch(cases, n, "mixed", tune=True)
+    for i, sp in enumerate(specs):
+        L = sp.velgrad
+        c = inputs.full_velocity_case(dims=sp.dims)
+        c.n_grains, c.seed = sp.n_grains, sp.seed
+        c.velgrad = (L[0, 0], L[1, 1], L[2, 2], L[1, 2], L[0, 2], L[0, 1])
+        path, base = write(tmp_path / ("o%d" % i), c)
+        ro, fo = oracle_run(path, base, n, fields=("sig",))
+        for k in range(n):
+            r = rel(rb[k][i]["sig_avg"], ro[k]["sig_avg"])
+            say(sp.dims, "incr", k + 1, "mixed batch vs oracle", r)
+            assert r < 1e-4
+        assert rel(fb[i]["sig"], fo["sig"]) < 1e-5
+
+
 def test_set_velgrad(tmp_path):
     say("set_velgrad")
     sp = _specs()[2]
